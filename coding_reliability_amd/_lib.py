@@ -1,0 +1,75 @@
+"""ctypes binding of libcodrel.so (the C ABI declared in include/coding_reliability.h).
+
+The HIP library is the only compute path: if libcodrel.so is missing or fails
+to load, every call raises.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG, "libcodrel.so")
+
+u64p = C.POINTER(C.c_uint64)
+u32p = C.POINTER(C.c_uint32)
+
+CR_OK = 0
+STATUS_NAMES = {
+    0: "CR_OK", 1: "CR_ERR_INVALID_ARGUMENT", 2: "CR_ERR_UNSUPPORTED", 3: "CR_ERR_NO_DEVICE",
+    4: "CR_ERR_OUT_OF_MEMORY", 5: "CR_ERR_DEVICE", 6: "CR_ERR_ADMISSION", 7: "CR_ERR_CANCELLED",
+    8: "CR_ERR_NOT_FOUND", 9: "CR_ERR_BUSY",
+}
+
+
+class CRError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class cr_device_info(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("compute_units", C.c_int),
+                ("max_clock_mhz", C.c_int), ("hbm_bytes", C.c_uint64)]
+
+
+class cr_pair_tuning(C.Structure):
+    _fields_ = [("rows_per_lane", C.c_uint32), ("chunk_blocks", C.c_uint32), ("waves", C.c_uint32)]
+
+
+_lib = None
+
+# name -> (restype, argtypes); every symbol declared in include/*.h is listed here
+SIGNATURES = {
+    "cr_status_string": (C.c_char_p, [C.c_int]),
+    "cr_last_error": (C.c_char_p, []),
+    "cr_version": (C.c_char_p, []),
+    "cr_context_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "cr_context_destroy": (C.c_int, [C.c_void_p]),
+    "cr_device_query": (C.c_int, [C.c_void_p, C.POINTER(cr_device_info)]),
+    "cr_an_pairs": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, u64p, C.c_uint32, C.POINTER(C.c_double)]),
+    "cr_an_pairs_tuned": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, u64p, C.c_uint32,
+                                    C.POINTER(cr_pair_tuning), C.POINTER(C.c_double)]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CRError(3, f"{LIB_PATH} is missing; build it with "
+                             "`python -m coding_reliability_amd.build` (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name, None)
+            if fn is None:
+                continue
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    if status != CR_OK:
+        raise CRError(status, lib().cr_last_error().decode(errors="replace"))

@@ -1,0 +1,227 @@
+// Context management, error reporting and the exhaustive-AN entry point.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "cr_internal.h"
+
+namespace cr {
+
+static thread_local char g_err[1024];
+
+cr_status fail(cr_status s, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return s;
+}
+
+void clear_error() { g_err[0] = 0; }
+
+cr_status DeviceBuffer::reserve(size_t want) {
+    if (want <= bytes) return CR_OK;
+    release();
+    size_t n = std::max(want, (size_t)256);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, n);
+    if (e != hipSuccess)
+        return fail(CR_ERR_OUT_OF_MEMORY, "hipMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+    ptr = p;
+    bytes = n;
+    return CR_OK;
+}
+
+void DeviceBuffer::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+cr_status HostBuffer::reserve(size_t want) {
+    if (want <= bytes) return CR_OK;
+    release();
+    size_t n = std::max(want, (size_t)4096);
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, n, hipHostMallocDefault);
+    if (e != hipSuccess)
+        return fail(CR_ERR_OUT_OF_MEMORY, "hipHostMalloc(%zu) failed: %s", n, hipGetErrorString(e));
+    ptr = p;
+    bytes = n;
+    return CR_OK;
+}
+
+void HostBuffer::release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+// Tuning for one pair job when the caller leaves a knob at 0: enough items
+// (about 16 per resident wave) to balance the persistent waves, rows per lane
+// large enough to amortise the per-item set-up.
+static void auto_pair_tuning(uint64_t N, uint32_t resident_waves, uint32_t &R, uint32_t &Q) {
+    if (R == 0) R = N >= (1ull << 16) ? 8 : (N >= (1ull << 11) ? 4 : 2);
+    if (Q == 0) {
+        const uint64_t RB = 64ull * R;
+        const uint64_t T = (N + RB - 1) / RB;
+        const uint64_t target = 16ull * std::max<uint32_t>(resident_waves, 1);
+        const long double q = (long double)T * (long double)T / (2.0L * (long double)target);
+        uint64_t qq = q < 1 ? 1 : (uint64_t)q;
+        // a lane adds at most R per column: keep R*CW well below 2^32
+        const uint64_t qmax = ((1ull << 30) / R) / RB;
+        Q = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(qq, 1), std::max<uint64_t>(qmax, 1));
+    }
+}
+
+cr_status run_pair_jobs(cr_context *ctx, std::vector<PairJob> &jobs, const PairLaunchConfig &cfg_in,
+                        double *kernel_ms) {
+    PairLaunchConfig cfg = cfg_in;
+    uint64_t total = 0;
+    for (auto &j : jobs) {
+        j.item_begin = total;
+        total += j.items;
+    }
+    CR_TRY(ctx->jobs.reserve(jobs.size() * sizeof(PairJob)));
+    CR_TRY(ctx->counters.reserve(sizeof(unsigned long long)));
+    CR_HIP(hipMemcpyAsync(ctx->jobs.ptr, jobs.data(), jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice,
+                          ctx->stream));
+    CR_TRY(launch_pairs(ctx->jobs.as<PairJob>(), (uint32_t)jobs.size(), total, ctx->counters.as<unsigned long long>(),
+                        cfg, ctx->cus, ctx->stream, kernel_ms ? ctx->ev0 : nullptr, kernel_ms ? ctx->ev1 : nullptr));
+    return CR_OK;
+}
+
+}  // namespace cr
+
+using namespace cr;
+
+extern "C" {
+
+const char *cr_status_string(cr_status s) {
+    switch (s) {
+    case CR_OK: return "ok";
+    case CR_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case CR_ERR_UNSUPPORTED: return "unsupported";
+    case CR_ERR_NO_DEVICE: return "no device";
+    case CR_ERR_OUT_OF_MEMORY: return "out of memory";
+    case CR_ERR_DEVICE: return "device error";
+    case CR_ERR_ADMISSION: return "admission refused";
+    case CR_ERR_CANCELLED: return "cancelled";
+    case CR_ERR_NOT_FOUND: return "not found";
+    case CR_ERR_BUSY: return "busy";
+    }
+    return "unknown status";
+}
+
+const char *cr_last_error(void) { return g_err; }
+
+const char *cr_version(void) { return "coding_reliability_amd 0.1 (gfx950)"; }
+
+cr_status cr_context_create(int device, cr_context **out) {
+    clear_error();
+    if (!out) return fail(CR_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(CR_ERR_NO_DEVICE, "no HIP device visible");
+    if (device < 0 || device >= n) return fail(CR_ERR_NO_DEVICE, "device %d not in [0, %d)", device, n);
+    hipDeviceProp_t prop;
+    CR_HIP(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(CR_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
+    CR_HIP(hipSetDevice(device));
+    cr_context *c = new cr_context();
+    c->device = device;
+    c->cus = prop.multiProcessorCount;
+    c->clock_mhz = prop.clockRate / 1000;
+    c->hbm = prop.totalGlobalMem;
+    snprintf(c->name, sizeof(c->name), "%s", prop.name);
+    snprintf(c->arch, sizeof(c->arch), "%s", prop.gcnArchName);
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+    if (e != hipSuccess) {
+        cr_context_destroy(c);
+        return fail(CR_ERR_DEVICE, "stream/event creation failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return CR_OK;
+}
+
+cr_status cr_context_destroy(cr_context *ctx) {
+    if (!ctx) return CR_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->jobs.release();
+    ctx->counters.release();
+    ctx->hist.release();
+    ctx->host.release();
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return CR_OK;
+}
+
+cr_status cr_device_query(cr_context *ctx, cr_device_info *out) {
+    clear_error();
+    if (!ctx || !out) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    memset(out, 0, sizeof(*out));
+    snprintf(out->name, sizeof(out->name), "%s", ctx->name);
+    snprintf(out->arch, sizeof(out->arch), "%s", ctx->arch);
+    out->compute_units = ctx->cus;
+    out->max_clock_mhz = ctx->clock_mhz;
+    out->hbm_bytes = ctx->hbm;
+    return CR_OK;
+}
+
+cr_status cr_an_pairs_tuned(cr_context *ctx, uint64_t A, uint64_t num_words, uint64_t *raw, uint32_t nbins,
+                            const cr_pair_tuning *tuning, double *kernel_ms) {
+    clear_error();
+    if (!ctx || !raw) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (A == 0 || num_words == 0) return fail(CR_ERR_INVALID_ARGUMENT, "A and num_words must be >= 1");
+    if ((unsigned __int128)(num_words - 1) * A >> 64)
+        return fail(CR_ERR_UNSUPPORTED, "codewords (num_words-1)*A exceed 64 bits");
+    const uint32_t need = pair_max_distance(A, num_words) + 1;
+    if (nbins < need) return fail(CR_ERR_INVALID_ARGUMENT, "nbins=%u but distances reach %u", nbins, need - 1);
+    CR_HIP(hipSetDevice(ctx->device));
+
+    uint32_t R = tuning ? tuning->rows_per_lane : 0, Q = tuning ? tuning->chunk_blocks : 0;
+    PairLaunchConfig cfg;
+    cfg.lds_rows = need + 1;
+    uint32_t Rr = R ? R : 8;
+    const uint32_t per_cu = pair_waves_per_cu(Rr, cfg.lds_rows);
+    auto_pair_tuning(num_words, per_cu * (uint32_t)ctx->cus, R, Q);
+    cfg.R = R;
+    cfg.waves = tuning ? tuning->waves : 0;
+
+    PairJob job{};
+    job.A = A;
+    job.N = num_words;
+    job.mode = pair_mode(A, num_words);
+    job.nbins = need;
+    pair_job_geometry(job, R, Q);
+
+    CR_TRY(ctx->hist.reserve(need * sizeof(uint64_t)));
+    job.out = ctx->hist.as<unsigned long long>();
+    CR_TRY(fill_zero_u64(job.out, need, ctx->stream));
+    std::vector<PairJob> jobs{job};
+    CR_TRY(run_pair_jobs(ctx, jobs, cfg, kernel_ms));
+    CR_TRY(ctx->host.reserve(need * sizeof(uint64_t)));
+    CR_HIP(hipMemcpyAsync(ctx->host.ptr, job.out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    CR_HIP(hipStreamSynchronize(ctx->stream));
+    memset(raw, 0, nbins * sizeof(uint64_t));
+    memcpy(raw, ctx->host.ptr, need * sizeof(uint64_t));
+    if (kernel_ms) {
+        float ms = 0;
+        CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *kernel_ms = ms;
+    }
+    return CR_OK;
+}
+
+cr_status cr_an_pairs(cr_context *ctx, uint64_t A, uint64_t num_words, uint64_t *raw, uint32_t nbins,
+                      double *kernel_ms) {
+    return cr_an_pairs_tuned(ctx, A, num_words, raw, nbins, nullptr, kernel_ms);
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "coding_reliability.h"
+
+namespace cr {
+
+// Records a message for cr_last_error() on this thread and returns s.
+cr_status fail(cr_status s, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+#define CR_HIP(call)                                                                                 \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return ::cr::fail(e_ == hipErrorOutOfMemory ? CR_ERR_OUT_OF_MEMORY : CR_ERR_DEVICE,      \
+                              "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__,       \
+                              __LINE__);                                                             \
+    } while (0)
+
+#define CR_TRY(expr)                                                                                 \
+    do {                                                                                             \
+        cr_status s_ = (expr);                                                                       \
+        if (s_ != CR_OK) return s_;                                                                  \
+    } while (0)
+
+// Device memory that grows on demand and is freed with its owner.
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    cr_status reserve(size_t want);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Pinned host memory that grows on demand.
+struct HostBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    cr_status reserve(size_t want);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// ---------------------------------------------------------------------------
+// Pair-histogram jobs (exhaustive AN and integer-stride grids).
+//
+// A job is the triangle {0 <= i < j < N} over codewords A*i.  Rows are cut in
+// blocks of RB = 64*R words (R codewords per lane held in registers); the
+// columns j >= b*RB of row block b are cut at multiples of CW = Q*RB into
+// items.  Item (b, c) covers columns [max(c*CW, b*RB), min((c+1)*CW, N)); the
+// part below (b+1)*RB is the diagonal block and is masked to j > i.
+// ---------------------------------------------------------------------------
+enum PairMode : uint32_t {
+    kModeW1 = 0,     // all codewords < 2^32: xor + bcnt per pair
+    kModeW2Seg = 1,  // 64-bit codewords, column high dword constant over long segments
+    kModeW2Gen = 2,  // 64-bit codewords, generic two-dword xor + bcnt
+};
+
+struct PairJob {
+    uint64_t A;
+    uint64_t N;
+    uint64_t item_begin;  // global index of the job's first item
+    uint64_t items;       // items of this job
+    uint64_t T;           // row blocks
+    uint64_t NC;          // column chunks
+    unsigned long long *out;  // device histogram, nbins entries, accumulated with atomics
+    uint32_t Q;           // chunk width in row blocks
+    uint32_t mode;
+    uint32_t nbins;       // bins flushed to out
+    uint32_t pad;
+};
+static_assert(sizeof(PairJob) == 72, "PairJob layout");
+
+// sum_{x=0}^{b-1} floor(x/Q)
+__host__ __device__ inline uint64_t pair_floor_sum(uint64_t b, uint64_t Q) {
+    const uint64_t q = b / Q, r = b - q * Q;
+    return Q * (q * (q - (q > 0 ? 1 : 0)) / 2) + r * q;
+}
+// items of row blocks [0, b)
+__host__ __device__ inline uint64_t pair_prefix(uint64_t b, uint64_t NC, uint64_t Q) {
+    return b * NC - pair_floor_sum(b, Q);
+}
+
+// Fills job geometry for R rows per lane and chunk width Q (in row blocks).
+void pair_job_geometry(PairJob &job, uint32_t R, uint32_t Q);
+// Bit length of the largest codeword (N-1)*A, i.e. the highest possible distance.
+uint32_t pair_max_distance(uint64_t A, uint64_t N);
+// Chooses the kernel mode for a job.
+uint32_t pair_mode(uint64_t A, uint64_t N);
+
+struct PairLaunchConfig {
+    uint32_t R = 8;
+    uint32_t waves = 0;     // 0 = resident capacity of the device
+    uint32_t lds_rows = 0;  // histogram rows in LDS (max nbins + 1 dummy row)
+};
+
+// Launches zero-fill + persistent pair kernel on `stream` for the device job
+// array d_jobs (njobs entries, item_begin prefix-summed), with `counter`
+// (device u64) as the work queue head.  Histograms jobs[i].out are NOT cleared
+// here.  Records ev0/ev1 around the pair kernel when non-null.
+cr_status launch_pairs(const PairJob *d_jobs, uint32_t njobs, uint64_t total_items,
+                       unsigned long long *counter, const PairLaunchConfig &cfg, int cus,
+                       hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1);
+
+// Resident single-wave workgroups per CU for the pair kernel with R rows per
+// lane and `lds_rows` histogram rows.
+uint32_t pair_waves_per_cu(uint32_t R, uint32_t lds_rows);
+
+// Zero-fills n 64-bit words on stream (a kernel, so it may be captured in graphs).
+cr_status fill_zero_u64(unsigned long long *p, size_t n, hipStream_t stream);
+
+}  // namespace cr
+
+struct cr_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int cus = 0;
+    int clock_mhz = 0;
+    uint64_t hbm = 0;
+    char name[64] = {0};
+    char arch[32] = {0};
+    cr::DeviceBuffer jobs;     // PairJob array
+    cr::DeviceBuffer counters; // work-queue heads
+    cr::DeviceBuffer hist;     // histograms
+    cr::HostBuffer host;       // pinned staging
+};
