@@ -10,9 +10,54 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import CRError, check, cr_device_info, cr_pair_tuning, lib
+from ._lib import (CRError, check, cr_an_code, cr_device_info, cr_ham_request, cr_job, cr_job_result,
+                   cr_pair_tuning, cr_server_config, cr_server_stats, lib)
 
-__all__ = ["Context", "CRError", "an_bins", "ceil_log2"]
+JOB_AN_EXHAUSTIVE, JOB_AN_GRID, JOB_HAMMING_MC = 0, 1, 2
+
+HAM_RANGE, HAM_STRIDE, HAM_LIST, HAM_MC = 0, 1, 2, 3
+
+__all__ = ["Context", "CRError", "an_bins", "ceil_log2", "grid_report", "hamming_parity_bits", "hamming_N",
+           "hamming_report", "HAM_RANGE", "HAM_STRIDE", "HAM_LIST", "HAM_MC"]
+
+
+def hamming_parity_bits(k: int) -> int:
+    return int(lib().cr_hamming_parity_bits(k))
+
+
+def hamming_N(k: int) -> int:
+    """Extended codeword length (data + parity + overall parity)."""
+    return k + hamming_parity_bits(k) + 1
+
+
+def hamming_report(k: int, weights: list[int], with_1bit: bool) -> list[int]:
+    """Reported counts of run_hamming (hamming.cu:182-208) from a non-extended
+    weight histogram (uint128 values)."""
+    N = hamming_N(k)
+    w = np.zeros(max(N + 1, len(weights)), dtype=np.uint64)
+    w[: len(weights)] = weights
+    out = np.zeros(2 * (N + 1), dtype=np.uint64)
+    check(lib().cr_hamming_report(k, _u64p(w), int(with_1bit), _u64p(out)))
+    return _u128(out)
+
+
+def _u64p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def _u128(buf) -> list[int]:
+    b = buf.reshape(-1, 2)
+    return [int(lo) | (int(hi) << 64) for lo, hi in b]
+
+
+def grid_report(gdim: int, n: int, m1: int, m2: int, hist: list[int], count_counts: int) -> list[int]:
+    """Scaled counts of run_ancoding_grid (an_coding_grid.cu:316-322) from a raw
+    grid histogram (uint128 values)."""
+    h = np.zeros(max(count_counts, len(hist)), dtype=np.uint64)
+    h[: len(hist)] = hist
+    out = np.zeros(2 * count_counts, dtype=np.uint64)
+    check(lib().cr_an_grid_report(gdim, n, m1, m2, _u64p(h), count_counts, _u64p(out)))
+    return _u128(out)
 
 
 def ceil_log2(A: int) -> int:
@@ -69,7 +114,162 @@ class Context:
             nbins = an_bins(A, num_words)
         out = np.zeros(nbins, dtype=np.uint64)
         ms = C.c_double(0)
-        t = cr_pair_tuning(rows_per_lane, chunk_blocks, waves)
-        check(lib().cr_an_pairs_tuned(self._h, A, num_words, out.ctypes.data_as(C.POINTER(C.c_uint64)), nbins,
-                                      C.byref(t), C.byref(ms)))
+        t = cr_pair_tuning(rows_per_lane, chunk_blocks, waves, 0)
+        check(lib().cr_an_pairs_tuned(self._h, A, num_words, _u64p(out), nbins, C.byref(t), C.byref(ms)))
         return [int(x) for x in out], ms.value
+
+    # --- batched exhaustive AN -------------------------------------------------
+    @staticmethod
+    def _codes(codes):
+        arr = (cr_an_code * len(codes))()
+        for i, (A, N) in enumerate(codes):
+            arr[i].A, arr[i].num_words = A, N
+        return arr
+
+    @staticmethod
+    def _stride(codes):
+        return max(an_bins(A, N) for A, N in codes)
+
+    def an_batch(self, codes, rows_per_lane: int = 0, items_per_wave: int = 0) -> tuple[list[list[int]], float]:
+        """Raw histograms of many (A, num_words) codes in one launch."""
+        stride = self._stride(codes)
+        out = np.zeros(len(codes) * stride, dtype=np.uint64)
+        ms = C.c_double(0)
+        t = cr_pair_tuning(rows_per_lane, 0, 0, items_per_wave)
+        check(lib().cr_an_batch(self._h, self._codes(codes), len(codes), _u64p(out), stride, C.byref(t),
+                                C.byref(ms)))
+        o = out.reshape(len(codes), stride)
+        return [[int(x) for x in row[: an_bins(A, N)]] for row, (A, N) in zip(o, codes)], ms.value
+
+    def batch_plan(self, codes, rows_per_lane: int = 0, items_per_wave: int = 0) -> "BatchPlan":
+        return BatchPlan(self, codes, rows_per_lane, items_per_wave)
+
+    def autotune_batch(self, codes, repeats: int = 1) -> tuple[dict, float]:
+        best = cr_pair_tuning()
+        ms = C.c_double(0)
+        check(lib().cr_autotune_batch(self._h, self._codes(codes), len(codes), repeats, C.byref(best), C.byref(ms)))
+        return dict(rows_per_lane=best.rows_per_lane, items_per_wave=best.items_per_wave), ms.value
+
+    def hamming_weights(self, k: int, source: int = HAM_RANGE, first: int = 0, count: int | None = None,
+                        stride: int = 0, words=None, seed: int = 0, stream: int = 0) -> tuple[list[int], float]:
+        """Non-extended codeword weight histogram over a stream of data words."""
+        lst = None
+        if source == HAM_LIST:
+            lst = np.ascontiguousarray(words, dtype=np.uint64)
+            count = len(lst)
+        if count is None:
+            count = 1 << k
+        req = cr_ham_request(k, source, first, count, stride, _u64p(lst) if lst is not None else None, seed,
+                             stream, 0)
+        nbins = hamming_N(k)
+        out = np.zeros(nbins, dtype=np.uint64)
+        ms = C.c_double(0)
+        check(lib().cr_hamming_weights(self._h, C.byref(req), _u64p(out), nbins, C.byref(ms)))
+        return [int(x) for x in out], ms.value
+
+    def an_mc(self, A: int, n: int, m: int, seed: int, nbins: int = 65) -> tuple[list[int], float]:
+        """Monte Carlo (word, sample) histogram of run_ancoding_mc with Philox samples."""
+        out = np.zeros(nbins, dtype=np.uint64)
+        ms = C.c_double(0)
+        check(lib().cr_an_mc(self._h, A, n, m, seed, _u64p(out), nbins, C.byref(ms)))
+        return [int(x) for x in out], ms.value
+
+    def an_grid_ref(self, gdim: int, n: int, m1: int, m2: int, A: int, nbins: int = 65) -> tuple[list[int], float]:
+        """Kernel-level histogram of the reference grid approximation
+        (run_ancoding_grid) and the kernel time in ms."""
+        out = np.zeros(nbins, dtype=np.uint64)
+        ms = C.c_double(0)
+        check(lib().cr_an_grid_ref(self._h, gdim, n, m1, m2, A, _u64p(out), nbins, C.byref(ms)))
+        return [int(x) for x in out], ms.value
+
+
+class BatchPlan:
+    """A captured hipGraph replaying one batch (cr_batch_plan)."""
+
+    def __init__(self, ctx: Context, codes, rows_per_lane: int = 0, items_per_wave: int = 0):
+        self.codes = list(codes)
+        self.stride = Context._stride(self.codes)
+        self._h = C.c_void_p()
+        t = cr_pair_tuning(rows_per_lane, 0, 0, items_per_wave)
+        check(lib().cr_batch_plan_create(ctx.handle, Context._codes(self.codes), len(self.codes), self.stride,
+                                         C.byref(t), C.byref(self._h)))
+        self._out = np.zeros(len(self.codes) * self.stride, dtype=np.uint64)
+
+    def run(self, fetch: bool = True):
+        ms = C.c_double(0)
+        check(lib().cr_batch_plan_run(self._h, _u64p(self._out) if fetch else None, C.byref(ms)))
+        if not fetch:
+            return None, ms.value
+        o = self._out.reshape(len(self.codes), self.stride)
+        return [[int(x) for x in row[: an_bins(A, N)]] for row, (A, N) in zip(o, self.codes)], ms.value
+
+    def close(self):
+        if self._h:
+            lib().cr_batch_plan_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching request loop (cr_server) on one context."""
+
+    def __init__(self, ctx: Context, hbm_budget_bytes: int = 0, max_active: int = 0, rows_per_lane: int = 0,
+                 iteration_ms: float = 0.0):
+        self._h = C.c_void_p()
+        cfg = cr_server_config(hbm_budget_bytes, max_active, rows_per_lane, iteration_ms)
+        check(lib().cr_server_create(ctx.handle, C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, kind: int, k: int, A: int = 0, stride: int = 0, samples: int = 0, seed: int = 0,
+               stream: int = 0) -> int:
+        j = cr_job(kind, k, A, stride, samples, seed, stream, 0)
+        rid = C.c_uint64()
+        check(lib().cr_server_submit(self._h, C.byref(j), C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: float = -1.0) -> None:
+        check(lib().cr_server_wait(self._h, rid, timeout_ms))
+
+    def result(self, rid: int) -> tuple[list[int], dict]:
+        res = cr_job_result()
+        out = np.zeros(130, dtype=np.uint64)
+        check(lib().cr_server_result(self._h, rid, _u64p(out), 130, C.byref(res)))
+        info = {f: getattr(res, f) for f, _ in cr_job_result._fields_}
+        return [int(x) for x in out[: res.nbins]], info
+
+    def state(self, rid: int) -> dict:
+        res = cr_job_result()
+        lib().cr_server_result(self._h, rid, None, 0, C.byref(res))
+        return {f: getattr(res, f) for f, _ in cr_job_result._fields_}
+
+    def cancel(self, rid: int) -> None:
+        check(lib().cr_server_cancel(self._h, rid))
+
+    def stats(self) -> dict:
+        st = cr_server_stats()
+        check(lib().cr_server_stats_get(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in cr_server_stats._fields_}
+
+    def now(self) -> float:
+        return lib().cr_server_now(self._h)
+
+    def close(self):
+        if self._h:
+            lib().cr_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -33,8 +33,41 @@ class cr_device_info(C.Structure):
                 ("max_clock_mhz", C.c_int), ("hbm_bytes", C.c_uint64)]
 
 
+class cr_ham_request(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("source", C.c_uint32), ("first", C.c_uint64), ("count", C.c_uint64),
+                ("stride", C.c_uint64), ("list", C.POINTER(C.c_uint64)), ("seed", C.c_uint64),
+                ("stream", C.c_uint32), ("reserved", C.c_uint32)]
+
+
 class cr_pair_tuning(C.Structure):
-    _fields_ = [("rows_per_lane", C.c_uint32), ("chunk_blocks", C.c_uint32), ("waves", C.c_uint32)]
+    _fields_ = [("rows_per_lane", C.c_uint32), ("chunk_blocks", C.c_uint32), ("waves", C.c_uint32),
+                ("items_per_wave", C.c_uint32)]
+
+
+class cr_job(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("k", C.c_uint32), ("A", C.c_uint64), ("stride", C.c_uint64),
+                ("samples", C.c_uint64), ("seed", C.c_uint64), ("stream", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class cr_server_config(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active", C.c_uint32), ("rows_per_lane", C.c_uint32),
+                ("iteration_ms", C.c_double)]
+
+
+class cr_job_result(C.Structure):
+    _fields_ = [("done", C.c_uint32), ("status", C.c_uint32), ("nbins", C.c_uint32), ("iterations", C.c_uint32),
+                ("submit_s", C.c_double), ("admit_s", C.c_double), ("finish_s", C.c_double),
+                ("units", C.c_uint64)]
+
+
+class cr_server_stats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("cancelled", C.c_uint64),
+                ("rejected", C.c_uint64), ("iterations", C.c_uint64), ("active", C.c_uint64),
+                ("queued", C.c_uint64), ("footprint_bytes", C.c_uint64), ("busy_device_ms", C.c_double)]
+
+
+class cr_an_code(C.Structure):
+    _fields_ = [("A", C.c_uint64), ("num_words", C.c_uint64)]
 
 
 _lib = None
@@ -50,6 +83,31 @@ SIGNATURES = {
     "cr_an_pairs": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, u64p, C.c_uint32, C.POINTER(C.c_double)]),
     "cr_an_pairs_tuned": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, u64p, C.c_uint32,
                                     C.POINTER(cr_pair_tuning), C.POINTER(C.c_double)]),
+    "cr_an_grid_ref": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, u64p,
+                                 C.c_uint32, C.POINTER(C.c_double)]),
+    "cr_an_grid_report": (C.c_int, [C.c_int, C.c_uint32, C.c_uint64, C.c_uint64, u64p, C.c_uint32, u64p]),
+    "cr_hamming_parity_bits": (C.c_uint32, [C.c_uint32]),
+    "cr_hamming_weights": (C.c_int, [C.c_void_p, C.POINTER(cr_ham_request), u64p, C.c_uint32,
+                                     C.POINTER(C.c_double)]),
+    "cr_hamming_report": (C.c_int, [C.c_uint32, u64p, C.c_int, u64p]),
+    "cr_an_batch": (C.c_int, [C.c_void_p, C.POINTER(cr_an_code), C.c_uint32, u64p, C.c_uint32,
+                              C.POINTER(cr_pair_tuning), C.POINTER(C.c_double)]),
+    "cr_batch_plan_create": (C.c_int, [C.c_void_p, C.POINTER(cr_an_code), C.c_uint32, C.c_uint32,
+                                       C.POINTER(cr_pair_tuning), C.POINTER(C.c_void_p)]),
+    "cr_batch_plan_run": (C.c_int, [C.c_void_p, u64p, C.POINTER(C.c_double)]),
+    "cr_batch_plan_destroy": (C.c_int, [C.c_void_p]),
+    "cr_autotune_batch": (C.c_int, [C.c_void_p, C.POINTER(cr_an_code), C.c_uint32, C.c_uint32,
+                                    C.POINTER(cr_pair_tuning), C.POINTER(C.c_double)]),
+    "cr_server_create": (C.c_int, [C.c_void_p, C.POINTER(cr_server_config), C.POINTER(C.c_void_p)]),
+    "cr_server_submit": (C.c_int, [C.c_void_p, C.POINTER(cr_job), C.POINTER(C.c_uint64)]),
+    "cr_server_result": (C.c_int, [C.c_void_p, C.c_uint64, u64p, C.c_uint32, C.POINTER(cr_job_result)]),
+    "cr_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, C.c_double]),
+    "cr_server_cancel": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "cr_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(cr_server_stats)]),
+    "cr_server_now": (C.c_double, [C.c_void_p]),
+    "cr_server_destroy": (C.c_int, [C.c_void_p]),
+    "cr_an_mc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, u64p, C.c_uint32,
+                           C.POINTER(C.c_double)]),
 }
 
 

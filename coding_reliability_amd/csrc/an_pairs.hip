@@ -16,44 +16,15 @@
 //     order, which is conflict free too; counters are flushed to the 64-bit
 //     device histogram with one atomic per bin per flush, before any 32-bit
 //     lane counter can wrap.
+#include "cr_device.h"
 #include "cr_internal.h"
 
 namespace cr {
 
 namespace {
 
-constexpr uint32_t kStride = 64;  // dwords between histogram rows in LDS (one per lane)
-constexpr uint32_t kUnroll = 4;   // columns per unrolled step
-
-// readfirstlane returns int: go through uint32_t so the low dword is not sign-extended
-__device__ __forceinline__ uint32_t uniform32(uint32_t x) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)x); }
-__device__ __forceinline__ uint64_t uniform64(uint64_t x) {
-    return ((uint64_t)uniform32((uint32_t)(x >> 32)) << 32) | (uint64_t)uniform32((uint32_t)x);
-}
-
-__device__ __forceinline__ void lds_inc(uint32_t *p) {
-    __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
-// Adds every lane column of rows [0, nbins) to out[] and clears the rows.
-// Lane l reduces row l (and l+64); at step s it reads column (s+l) mod 64, so
-// the 64 lanes of one read hit 64 different banks.
-__device__ void flush_rows(uint32_t *lds, uint32_t nbins, unsigned long long *out) {
-    __syncthreads();
-    const uint32_t lane = threadIdx.x;
-    for (uint32_t b = lane; b < nbins; b += 64) {
-        uint32_t *row = lds + b * kStride;
-        unsigned long long s = 0;
-#pragma unroll 8
-        for (uint32_t step = 0; step < 64; ++step) {
-            const uint32_t l = (step + lane) & 63u;
-            s += row[l];
-            row[l] = 0;
-        }
-        if (s) atomicAdd(out + b, s);
-    }
-    __syncthreads();
-}
+using namespace dev;
+constexpr uint32_t kUnroll = 4;  // columns per unrolled step
 
 template <int R>
 struct Rows {
@@ -177,8 +148,7 @@ __global__ __launch_bounds__(64) void pair_kernel(const PairJob *__restrict__ jo
                                                   uint32_t lds_rows) {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
-    for (uint32_t b = 0; b < lds_rows; ++b) lds[b * kStride + lane] = 0;
-    __syncthreads();
+    lds_clear(lds, lds_rows);
     uint32_t *const lane_col = lds + lane;
     const uint32_t dummy = lds_rows - 1;
     constexpr uint64_t RB = 64ull * R;
@@ -187,9 +157,7 @@ __global__ __launch_bounds__(64) void pair_kernel(const PairJob *__restrict__ jo
     uint64_t pending = 0;        // columns accumulated since the last flush
 
     for (;;) {
-        uint64_t t = 0;
-        if (lane == 0) t = atomicAdd(counter, 1ull);
-        t = uniform64(t);
+        const uint64_t t = next_item(counter);
         if (t >= total_items) break;
 
         // job lookup: current job first, else binary search over item_begin
@@ -206,7 +174,7 @@ __global__ __launch_bounds__(64) void pair_kernel(const PairJob *__restrict__ jo
         }
         const PairJob &job = jobs[jb];
         const uint64_t A = job.A, N = job.N, NC = job.NC, Q = job.Q;
-        const uint64_t tl = t - job.item_begin;
+        const uint64_t tl = t - job.item_begin + job.item_offset;
         // row block: largest b with prefix(b) <= tl
         uint64_t lo = 0, hi = job.T - 1;
         while (lo < hi) {
@@ -225,7 +193,7 @@ __global__ __launch_bounds__(64) void pair_kernel(const PairJob *__restrict__ jo
         j1 = uniform64(j1);
 
         if (jb != cur || (pending + (j1 - j0)) * R >= 0xFFFFFFFFull) {
-            if (cur != 0xFFFFFFFFu) flush_rows(lds, jobs[cur].nbins, jobs[cur].out);
+            if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, jobs[cur].out);
             cur = jb;
             pending = 0;
         }
@@ -249,7 +217,7 @@ __global__ __launch_bounds__(64) void pair_kernel(const PairJob *__restrict__ jo
                 pairs_w2gen<R>(lane_col, rw, A, jr0, j1);
         }
     }
-    if (cur != 0xFFFFFFFFu) flush_rows(lds, jobs[cur].nbins, jobs[cur].out);
+    if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, jobs[cur].out);
 }
 
 template <int R>

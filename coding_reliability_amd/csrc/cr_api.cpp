@@ -154,6 +154,8 @@ cr_status cr_context_destroy(cr_context *ctx) {
     ctx->jobs.release();
     ctx->counters.release();
     ctx->hist.release();
+    ctx->table_a.release();
+    ctx->table_b.release();
     ctx->host.release();
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

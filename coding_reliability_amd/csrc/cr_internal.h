@@ -67,8 +67,9 @@ enum PairMode : uint32_t {
 struct PairJob {
     uint64_t A;
     uint64_t N;
-    uint64_t item_begin;  // global index of the job's first item
-    uint64_t items;       // items of this job
+    uint64_t item_begin;  // global index of the job's first item in this launch
+    uint64_t items;       // items of this job in this launch
+    uint64_t item_offset; // local index of the first item (a launch may run a slice)
     uint64_t T;           // row blocks
     uint64_t NC;          // column chunks
     unsigned long long *out;  // device histogram, nbins entries, accumulated with atomics
@@ -77,7 +78,7 @@ struct PairJob {
     uint32_t nbins;       // bins flushed to out
     uint32_t pad;
 };
-static_assert(sizeof(PairJob) == 72, "PairJob layout");
+static_assert(sizeof(PairJob) == 80, "PairJob layout");
 
 // sum_{x=0}^{b-1} floor(x/Q)
 __host__ __device__ inline uint64_t pair_floor_sum(uint64_t b, uint64_t Q) {
@@ -114,6 +115,47 @@ cr_status launch_pairs(const PairJob *d_jobs, uint32_t njobs, uint64_t total_ite
 // lane and `lds_rows` histogram rows.
 uint32_t pair_waves_per_cu(uint32_t R, uint32_t lds_rows);
 
+// Rectangle histogram rows x column table (see rect_pairs.hip).
+struct RectLaunch {
+    uint64_t A = 1, row0 = 0, nrows = 0;
+    const uint64_t *rows = nullptr;  // device row table, or null for rows A*(row0+i)
+    const uint64_t *cols = nullptr;  // device column table
+    uint64_t ncols = 0;
+    uint32_t mask32 = 0;             // truncate generated rows to 32 bits
+    bool w2 = false;                 // some codeword exceeds 32 bits
+    unsigned long long *counter = nullptr;
+    unsigned long long *out = nullptr;
+    uint32_t nbins = 0, first_bin = 0, lds_rows = 0;
+};
+cr_status launch_rect(const RectLaunch &L, hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1, int cus);
+
+// Hamming weight jobs (see hamming.hip).
+enum HamSource : uint32_t { kHamRange = 0, kHamStride = 1, kHamList = 2, kHamMc = 3 };
+struct HamJob {
+    uint64_t first;            // range/stride offset, or first Monte Carlo sample index
+    uint64_t stride;
+    uint64_t count;            // samples
+    const uint64_t *list;      // device list (kHamList)
+    uint64_t seed;
+    uint64_t item_begin, items;  // global range of this job in the launch
+    uint64_t item_offset;        // local index of the first item (slices)
+    unsigned long long *out;   // device weight histogram (nbins entries)
+    uint64_t masks[7];         // parity masks truncated to k bits, zero when unused
+    uint32_t source, k, stream, nbins;
+};
+static_assert(sizeof(HamJob) == 144, "HamJob layout");
+
+// Parity masks of the (N,k) extended Hamming codes, truncated to k bits;
+// returns the number of non-overall parity bits.
+uint32_t hamming_masks(uint32_t k, uint64_t masks[7]);
+uint64_t hamming_items(uint64_t count);
+cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_items, bool w64,
+                         unsigned long long *counter, uint32_t lds_rows, int cus, hipStream_t stream, hipEvent_t ev0,
+                         hipEvent_t ev1);
+cr_status launch_an_mc(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, unsigned long long *counter,
+                       unsigned long long *out, uint32_t nbins, int cus, hipStream_t stream, hipEvent_t ev0,
+                       hipEvent_t ev1);
+
 // Zero-fills n 64-bit words on stream (a kernel, so it may be captured in graphs).
 cr_status fill_zero_u64(unsigned long long *p, size_t n, hipStream_t stream);
 
@@ -131,5 +173,7 @@ struct cr_context {
     cr::DeviceBuffer jobs;     // PairJob array
     cr::DeviceBuffer counters; // work-queue heads
     cr::DeviceBuffer hist;     // histograms
+    cr::DeviceBuffer table_a;  // row / sample tables
+    cr::DeviceBuffer table_b;
     cr::HostBuffer host;       // pinned staging
 };

@@ -1,0 +1,231 @@
+// Extended-Hamming weight histograms and AN Monte Carlo pair histograms (gfx950).
+//
+// Hamming: by linearity the distance distribution of the code is the weight
+// distribution of its codewords (the reference's method,
+// distance_distribution/src/hamming.cu:183-208), so the kernel histograms the
+// weight of the non-extended codeword, popcount(d) + sum_t parity(d & mask_t)
+// (computeHamming<N>, hamming.cu:39-59), over a stream of data words d:
+//   range   d = first + s                       (exhaustive: dhamming, hamming.cu:64-81)
+//   stride  d = (first + s*stride) mod 2^k      (integer grid)
+//   list    d = list[s]                         (the reference grid's sample points)
+//   mc      d = Philox word s of (seed, stream) (replaces dhamming_mc's cuRAND draw)
+// AN Monte Carlo: every word A*i against m Philox samples A*u_ij
+// (replaces dancoding_mc_shared, an_coding_mc.cu:94-158).
+#include "cr_device.h"
+#include "cr_internal.h"
+
+namespace cr {
+namespace {
+
+using namespace dev;
+
+constexpr uint32_t kSamplesPerLane = 64;  // per item
+constexpr uint64_t kItemSamples = 64ull * kSamplesPerLane;
+
+template <bool W64>
+__device__ __forceinline__ uint32_t weight_nonext(uint64_t d, const uint64_t (&m)[7]) {
+    if (W64) {
+        uint32_t w = __popcll(d);
+#pragma unroll
+        for (int t = 0; t < 7; ++t) w += __popcll(d & m[t]) & 1u;
+        return w;
+    } else {
+        const uint32_t x = (uint32_t)d;
+        uint32_t w = __popc(x);
+#pragma unroll
+        for (int t = 0; t < 6; ++t) w += __popc(x & (uint32_t)m[t]) & 1u;
+        return w;
+    }
+}
+
+__device__ __forceinline__ uint32_t find_job(const HamJob *jobs, uint32_t njobs, uint64_t t, uint32_t cur) {
+    if (cur != 0xFFFFFFFFu && t >= jobs[cur].item_begin && t < jobs[cur].item_begin + jobs[cur].items) return cur;
+    uint32_t lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].item_begin <= t) lo = mid; else hi = mid - 1;
+    }
+    return uniform32(lo);
+}
+
+template <bool W64>
+__global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs, uint32_t njobs, uint64_t total_items,
+                                                 unsigned long long *__restrict__ counter, uint32_t lds_rows) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = threadIdx.x;
+    lds_clear(lds, lds_rows);
+    uint32_t *const lane_col = lds + lane;
+    const uint32_t dummy = lds_rows - 1;
+    uint32_t cur = 0xFFFFFFFFu;
+    uint64_t pending = 0;
+
+    for (;;) {
+        const uint64_t t = next_item(counter);
+        if (t >= total_items) break;
+        const uint32_t jb = find_job(jobs, njobs, t, cur);
+        const HamJob &job = jobs[jb];
+        if (jb != cur || pending + kSamplesPerLane >= 0xFFFFFFFFull) {
+            if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, jobs[cur].out);
+            cur = jb;
+            pending = 0;
+        }
+        pending += kSamplesPerLane;
+        uint64_t m[7];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) m[i] = job.masks[i];
+        const uint64_t dmask = job.k >= 64 ? ~0ull : ((1ull << job.k) - 1);
+        const uint64_t base = (t - job.item_begin + job.item_offset) * kItemSamples;  // first sample of the item
+        const uint64_t count = job.count;
+
+        if (job.source == kHamMc) {
+            const uint32_t per = job.k <= 32 ? 4 : 2;  // words per Philox block
+            const uint32_t k0 = (uint32_t)job.seed, k1 = (uint32_t)(job.seed >> 32);
+            for (uint32_t q = 0; q < kSamplesPerLane / per; ++q) {
+                const uint64_t s = base + ((uint64_t)q * 64 + lane) * per;  // first local sample of this lane
+                const uint64_t blk = (job.first + s) / per;
+                uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), job.stream, kMcTag};
+                philox4x32_10(c, k0, k1);
+                if (per == 4) {
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        const uint32_t w = weight_nonext<W64>((uint64_t)c[u] & dmask, m);
+                        lds_inc(lane_col + (s + u < count ? w : dummy) * kStride);
+                    }
+                } else {
+#pragma unroll
+                    for (uint32_t u = 0; u < 2; ++u) {
+                        const uint64_t d = ((uint64_t)c[2 * u] | ((uint64_t)c[2 * u + 1] << 32)) & dmask;
+                        const uint32_t w = weight_nonext<W64>(d, m);
+                        lds_inc(lane_col + (s + u < count ? w : dummy) * kStride);
+                    }
+                }
+            }
+        } else {
+            for (uint32_t q = 0; q < kSamplesPerLane; ++q) {
+                const uint64_t s = base + (uint64_t)q * 64 + lane;
+                const bool ok = s < count;
+                uint64_t d;
+                if (job.source == kHamRange)
+                    d = job.first + s;
+                else if (job.source == kHamStride)
+                    d = job.first + s * job.stride;
+                else
+                    d = ok ? job.list[s] : 0;
+                const uint32_t w = weight_nonext<W64>(d & dmask, m);
+                lds_inc(lane_col + (ok ? w : dummy) * kStride);
+            }
+        }
+    }
+    if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, jobs[cur].out);
+}
+
+// AN Monte Carlo (n <= 32): item = (row block of 64*RM words, sample chunk
+// [j0, j0+CJ)).  Row i draws u_ij = word j of the Philox stream (seed, i):
+// block j/4 of counter (j/4, 0, i, tag), word j%4, masked to n bits; the
+// codewords A*i and A*u_ij are compared.
+constexpr int RM = 4;
+constexpr uint32_t CJ = 256;  // samples per item (multiple of 4)
+
+__global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint64_t RBlocks,
+                                                   uint64_t SC, uint64_t total_items,
+                                                   unsigned long long *__restrict__ counter,
+                                                   unsigned long long *__restrict__ out, uint32_t nbins,
+                                                   uint32_t lds_rows) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t lane = threadIdx.x;
+    lds_clear(lds, lds_rows);
+    uint32_t *const lane_col = lds + lane;
+    const uint32_t dummy = lds_rows - 1;
+    const uint64_t nwords = 1ull << n;
+    const uint32_t dmask = (uint32_t)(nwords - 1);
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    uint64_t pending = 0;
+    bool dirty = false;
+    for (;;) {
+        const uint64_t t = next_item(counter);
+        if (t >= total_items) break;
+        const uint64_t rb = uniform64(t / SC), sc = uniform64(t - (t / SC) * SC);
+        const uint64_t j0 = sc * CJ;
+        const uint64_t j1 = j0 + CJ < m ? j0 + CJ : m;
+        if ((pending + (j1 - j0)) * RM >= 0xFFFFFFFFull) {
+            flush_rows(lds, 0, nbins, out);
+            pending = 0;
+        }
+        pending += j1 - j0;
+        dirty = true;
+#pragma unroll
+        for (int r = 0; r < RM; ++r) {
+            const uint64_t i = rb * (64ull * RM) + (uint64_t)r * 64 + lane;
+            const bool ok = i < nwords;
+            const uint64_t w = A * i;
+            for (uint64_t j = j0; j < j1; j += 4) {
+                const uint64_t blk = j >> 2;
+                uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)i, kMcTag};
+                philox4x32_10(c, k0, k1);
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const uint32_t d = (uint32_t)__popcll(w ^ (A * (uint64_t)(c[u] & dmask)));
+                    lds_inc(lane_col + ((ok && j + u < j1) ? d : dummy) * kStride);
+                }
+            }
+        }
+    }
+    if (dirty) flush_rows(lds, 0, nbins, out);
+}
+
+}  // namespace
+
+cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_items, bool w64,
+                         unsigned long long *counter, uint32_t lds_rows, int cus, hipStream_t stream, hipEvent_t ev0,
+                         hipEvent_t ev1) {
+    const void *k = w64 ? reinterpret_cast<const void *>(&ham_kernel<true>)
+                        : reinterpret_cast<const void *>(&ham_kernel<false>);
+    if (lds_rows < 2 || lds_rows > 131) return fail(CR_ERR_INVALID_ARGUMENT, "bad histogram rows %u", lds_rows);
+    const size_t lds = (size_t)lds_rows * kStride * sizeof(uint32_t);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, lds) != hipSuccess || per_cu <= 0)
+        return fail(CR_ERR_DEVICE, "occupancy query failed for the Hamming kernel");
+    uint64_t waves = (uint64_t)per_cu * cus;
+    if (waves > total_items) waves = total_items;
+    if (waves == 0) waves = 1;
+    CR_TRY(fill_zero_u64(counter, 1, stream));
+    if (ev0) CR_HIP(hipEventRecord(ev0, stream));
+    const HamJob *a_jobs = d_jobs;
+    uint32_t a_nj = njobs, a_rows = lds_rows;
+    uint64_t a_items = total_items;
+    unsigned long long *a_counter = counter;
+    void *args[] = {(void *)&a_jobs, &a_nj, &a_items, &a_counter, &a_rows};
+    CR_HIP(hipLaunchKernel(k, dim3((uint32_t)waves), dim3(64), args, lds, stream));
+    if (ev1) CR_HIP(hipEventRecord(ev1, stream));
+    return CR_OK;
+}
+
+uint64_t hamming_items(uint64_t count) { return (count + kItemSamples - 1) / kItemSamples; }
+
+cr_status launch_an_mc(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, unsigned long long *counter,
+                       unsigned long long *out, uint32_t nbins, int cus, hipStream_t stream, hipEvent_t ev0,
+                       hipEvent_t ev1) {
+    const uint32_t lds_rows = nbins + 1;
+    if (lds_rows > 66) return fail(CR_ERR_INVALID_ARGUMENT, "bad histogram rows %u", lds_rows);
+    const void *k = reinterpret_cast<const void *>(&an_mc_kernel);
+    const size_t lds = (size_t)lds_rows * kStride * sizeof(uint32_t);
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, lds) != hipSuccess || per_cu <= 0)
+        return fail(CR_ERR_DEVICE, "occupancy query failed for the AN Monte Carlo kernel");
+    const uint64_t RBlocks = ((1ull << n) + 64ull * RM - 1) / (64ull * RM);
+    const uint64_t SC = (m + CJ - 1) / CJ;
+    const uint64_t items = RBlocks * SC;
+    uint64_t waves = (uint64_t)per_cu * cus;
+    if (waves > items) waves = items;
+    CR_TRY(fill_zero_u64(counter, 1, stream));
+    if (ev0) CR_HIP(hipEventRecord(ev0, stream));
+    uint64_t a_A = A, a_m = m, a_seed = seed, a_RB = RBlocks, a_SC = SC, a_items = items;
+    uint32_t a_n = n, a_nbins = nbins, a_rows = lds_rows;
+    unsigned long long *a_counter = counter, *a_out = out;
+    void *args[] = {&a_A, &a_n, &a_m, &a_seed, &a_RB, &a_SC, &a_items, &a_counter, &a_out, &a_nbins, &a_rows};
+    CR_HIP(hipLaunchKernel(k, dim3((uint32_t)waves), dim3(64), args, lds, stream));
+    if (ev1) CR_HIP(hipEventRecord(ev1, stream));
+    return CR_OK;
+}
+
+}  // namespace cr

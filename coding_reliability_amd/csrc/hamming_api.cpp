@@ -1,0 +1,154 @@
+// Host side of the Extended Hamming and AN Monte Carlo entry points.
+#include <cstring>
+#include <vector>
+
+#include "cr_internal.h"
+
+namespace cr {
+
+// The reference's (39,32) parity masks (distance_distribution/src/hamming.h:54-64)
+// are the low halves of the (72,64) patterns of legacy/Hamming/ExtHamming64.hpp:31-37;
+// one table truncated to the data width serves every k.
+static const uint64_t kHamPatterns[7] = {
+    0xAB55555556AAAD5Bull, 0xCD9999999B33366Dull, 0x78F1E1E1E3C3C78Eull, 0x01FE01FE03FC07F0ull,
+    0x01FFFE0003FFF800ull, 0x01FFFFFFFC000000ull, 0xFE00000000000000ull};
+
+uint32_t hamming_masks(uint32_t k, uint64_t masks[7]) {
+    const uint64_t dm = k >= 64 ? ~0ull : ((1ull << k) - 1);
+    uint32_t r = 0;
+    for (int t = 0; t < 7; ++t) {
+        masks[t] = kHamPatterns[t] & dm;
+        if (masks[t]) ++r;
+    }
+    return r;
+}
+
+}  // namespace cr
+
+using namespace cr;
+
+extern "C" uint32_t cr_hamming_parity_bits(uint32_t k) {
+    uint64_t m[7];
+    return hamming_masks(k, m);
+}
+
+extern "C" cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *req, uint64_t *hist, uint32_t nbins,
+                                        double *kernel_ms) {
+    clear_error();
+    if (!ctx || !req || !hist) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (req->k < 1 || req->k > 64) return fail(CR_ERR_INVALID_ARGUMENT, "k=%u outside [1, 64]", req->k);
+    if (req->source > CR_HAM_MC) return fail(CR_ERR_INVALID_ARGUMENT, "unknown source %u", req->source);
+    if (req->source == CR_HAM_LIST && req->count && !req->list)
+        return fail(CR_ERR_INVALID_ARGUMENT, "list source without a list");
+    const uint32_t per = req->k <= 32 ? 4 : 2;
+    if (req->source == CR_HAM_MC && req->first % per)
+        return fail(CR_ERR_INVALID_ARGUMENT, "Monte Carlo first sample must be a multiple of %u", per);
+    HamJob job{};
+    const uint32_t r = hamming_masks(req->k, job.masks);
+    const uint32_t need = req->k + r + 1;  // weights 0..k+r
+    if (nbins < need) return fail(CR_ERR_INVALID_ARGUMENT, "nbins=%u but weights reach %u", nbins, need - 1);
+    CR_HIP(hipSetDevice(ctx->device));
+    memset(hist, 0, nbins * sizeof(uint64_t));
+    if (req->count == 0) {
+        if (kernel_ms) *kernel_ms = 0;
+        return CR_OK;
+    }
+    job.first = req->first;
+    job.stride = req->stride;
+    job.count = req->count;
+    job.seed = req->seed;
+    job.source = req->source;
+    job.k = req->k;
+    job.stream = req->stream;
+    job.nbins = need;
+    job.item_begin = 0;
+    job.items = hamming_items(req->count);
+    if (req->source == CR_HAM_LIST) {
+        CR_TRY(ctx->table_a.reserve(req->count * sizeof(uint64_t)));
+        CR_HIP(hipMemcpyAsync(ctx->table_a.ptr, req->list, req->count * sizeof(uint64_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+        job.list = ctx->table_a.as<uint64_t>();
+    }
+    CR_TRY(ctx->hist.reserve(need * sizeof(uint64_t)));
+    CR_TRY(ctx->counters.reserve(sizeof(uint64_t)));
+    CR_TRY(ctx->jobs.reserve(sizeof(HamJob)));
+    job.out = ctx->hist.as<unsigned long long>();
+    CR_TRY(fill_zero_u64(job.out, need, ctx->stream));
+    CR_HIP(hipMemcpyAsync(ctx->jobs.ptr, &job, sizeof(job), hipMemcpyHostToDevice, ctx->stream));
+    CR_TRY(launch_hamming(ctx->jobs.as<HamJob>(), 1, job.items, req->k > 32, ctx->counters.as<unsigned long long>(),
+                          need + 1, ctx->cus, ctx->stream, kernel_ms ? ctx->ev0 : nullptr,
+                          kernel_ms ? ctx->ev1 : nullptr));
+    CR_TRY(ctx->host.reserve(need * sizeof(uint64_t)));
+    CR_HIP(hipMemcpyAsync(ctx->host.ptr, job.out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    CR_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(hist, ctx->host.ptr, need * sizeof(uint64_t));
+    if (kernel_ms) {
+        float ms = 0;
+        CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *kernel_ms = ms;
+    }
+    return CR_OK;
+}
+
+extern "C" cr_status cr_hamming_report(uint32_t k, const uint64_t *weights, int with_1bit, uint64_t *out_u128) {
+    clear_error();
+    if (!weights || !out_u128) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (k < 1 || k > 64) return fail(CR_ERR_INVALID_ARGUMENT, "k=%u outside [1, 64]", k);
+    if (k + cr_hamming_parity_bits(k) + 1 > 121) return fail(CR_ERR_UNSUPPORTED, "codeword too long");
+    const uint32_t N = k + cr_hamming_parity_bits(k) + 1;  // codeword bits incl. overall parity
+    const uint32_t cc = N + 1;
+    std::vector<unsigned __int128> c(cc, 0);
+    // hamming.cu:183-208
+    c[0] = (unsigned __int128)1 << k;
+    c[1] = with_1bit ? ((unsigned __int128)1 << k) * N : 0;
+    c[2] = 0;
+    for (uint32_t i = 4; i < cc; i += 2) {
+        c[i] = (unsigned __int128)weights[i] + weights[i - 1];
+        c[i - 1] = 0;
+    }
+    if (with_1bit)
+        for (uint32_t i = 3; i < cc; i += 2)
+            c[i] = i + 1 < cc ? (unsigned __int128)(i + 1) * c[i + 1] + (unsigned __int128)(N - i + 1) * c[i - 1]
+                              : (unsigned __int128)(N - i + 1) * c[i - 1];
+    for (uint32_t i = 3; i < cc; ++i) c[i] <<= k;
+    for (uint32_t i = 0; i < cc; ++i) {
+        out_u128[2 * i] = (uint64_t)c[i];
+        out_u128[2 * i + 1] = (uint64_t)(c[i] >> 64);
+    }
+    return CR_OK;
+}
+
+extern "C" cr_status cr_an_mc(cr_context *ctx, uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint64_t *hist,
+                              uint32_t nbins, double *kernel_ms) {
+    clear_error();
+    if (!ctx || !hist) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (A == 0) return fail(CR_ERR_INVALID_ARGUMENT, "A must be >= 1");
+    if (n < 1 || n > 32) return fail(CR_ERR_UNSUPPORTED, "AN Monte Carlo supports 1 <= n <= 32 (got %u)", n);
+    if ((unsigned __int128)A << n >> 64) return fail(CR_ERR_UNSUPPORTED, "A << n exceeds 64 bits");
+    uint32_t bits = 0;
+    for (unsigned __int128 x = (unsigned __int128)A * ((1ull << n) - 1); x; x >>= 1) ++bits;
+    const uint32_t need = bits + 1;
+    if (nbins < need) return fail(CR_ERR_INVALID_ARGUMENT, "nbins=%u but distances reach %u", nbins, need - 1);
+    CR_HIP(hipSetDevice(ctx->device));
+    memset(hist, 0, nbins * sizeof(uint64_t));
+    if (m == 0) {
+        if (kernel_ms) *kernel_ms = 0;
+        return CR_OK;
+    }
+    CR_TRY(ctx->hist.reserve(need * sizeof(uint64_t)));
+    CR_TRY(ctx->counters.reserve(sizeof(uint64_t)));
+    unsigned long long *out = ctx->hist.as<unsigned long long>();
+    CR_TRY(fill_zero_u64(out, need, ctx->stream));
+    CR_TRY(launch_an_mc(A, n, m, seed, ctx->counters.as<unsigned long long>(), out, need, ctx->cus, ctx->stream,
+                        kernel_ms ? ctx->ev0 : nullptr, kernel_ms ? ctx->ev1 : nullptr));
+    CR_TRY(ctx->host.reserve(need * sizeof(uint64_t)));
+    CR_HIP(hipMemcpyAsync(ctx->host.ptr, out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    CR_HIP(hipStreamSynchronize(ctx->stream));
+    memcpy(hist, ctx->host.ptr, need * sizeof(uint64_t));
+    if (kernel_ms) {
+        float ms = 0;
+        CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *kernel_ms = ms;
+    }
+    return CR_OK;
+}

@@ -1,0 +1,480 @@
+// Continuous-batching request loop with device-resident histogram state.
+//
+// One scheduler thread per server.  Each iteration:
+//   1. admits queued requests (FIFO) while a slot is free and the summed
+//      device footprint stays within the HBM budget; an admitted request gets
+//      a histogram slot in the server's device slab, zeroed on the device;
+//   2. gives every active request a slice of its remaining work items, sized
+//      from a measured throughput model so the iteration lasts about
+//      iteration_ms of device time, shared equally among active requests;
+//   3. launches one pair-kernel pass over the AN slices and one Hamming pass
+//      over the Monte Carlo slices (each kernel works on all slices at once);
+//   4. retires finished requests: copies their histogram to the host and
+//      frees the slot.
+// Requests arriving during an iteration are admitted at the next one.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "cr_internal.h"
+
+using namespace cr;
+
+namespace {
+
+constexpr uint32_t kSlotBins = 130;  // histogram slot size (uint64 counters)
+
+struct Request {
+    uint64_t id = 0;
+    cr_job job{};
+    uint32_t kind = 0;
+    cr_status status = CR_OK;
+    int state = 0;  // 0 queued, 1 active, 2 done
+    bool cancel = false;
+    int slot = -1;
+    uint64_t footprint = 0;
+    uint32_t nbins = 0;
+    uint64_t total_items = 0, cursor = 0;
+    long double units_per_item = 0;
+    uint64_t units = 0;
+    PairJob pj{};
+    HamJob hj{};
+    uint32_t iterations = 0;
+    double t_submit = 0, t_admit = 0, t_finish = 0;
+    std::vector<uint64_t> result;
+};
+
+}  // namespace
+
+struct cr_server {
+    cr_context *ctx = nullptr;
+    cr_server_config cfg{};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DeviceBuffer slab, pair_jobs, ham_jobs, counter;
+    HostBuffer staging, readback;
+    std::vector<int> free_slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<uint64_t> queue;
+    std::map<uint64_t, std::unique_ptr<Request>> reqs;
+    std::vector<uint64_t> active;
+    uint64_t next_id = 1;
+    uint64_t footprint = 0;
+    cr_server_stats st{};
+    bool stop = false;
+    std::thread worker;
+    std::chrono::steady_clock::time_point t0;
+    long double pair_rate = 5e9L;    // pairs per device ms (refined online)
+    long double sample_rate = 2e9L;  // Hamming samples per device ms
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void loop();
+    cr_status iterate(std::vector<Request *> &run);
+};
+
+namespace {
+
+// Validates a job and fills its work description.
+cr_status describe(cr_server *s, const cr_job &j, Request &r) {
+    r.job = j;
+    r.kind = j.kind;
+    const uint32_t R = s->cfg.rows_per_lane;
+    if (j.kind == CR_JOB_AN_EXHAUSTIVE || j.kind == CR_JOB_AN_GRID) {
+        if (j.k < 1 || j.k > 40) return fail(CR_ERR_UNSUPPORTED, "AN request k=%u outside [1, 40]", j.k);
+        if (j.A == 0) return fail(CR_ERR_INVALID_ARGUMENT, "A must be >= 1");
+        uint64_t A = j.A, N = 1ull << j.k;
+        if (j.kind == CR_JOB_AN_GRID) {
+            if (j.stride == 0) return fail(CR_ERR_INVALID_ARGUMENT, "grid stride must be >= 1");
+            if ((unsigned __int128)A * j.stride >> 64) return fail(CR_ERR_UNSUPPORTED, "A*stride exceeds 64 bits");
+            A = j.A * j.stride;
+            N = ((1ull << j.k) + j.stride - 1) / j.stride;
+        }
+        if ((unsigned __int128)(N - 1) * A >> 64) return fail(CR_ERR_UNSUPPORTED, "codewords exceed 64 bits");
+        PairJob &p = r.pj;
+        p = PairJob{};
+        p.A = A;
+        p.N = N;
+        p.mode = pair_mode(A, N);
+        p.nbins = pair_max_distance(A, N) + 1;
+        // items of about 2^27 pairs
+        const uint64_t RB = 64ull * R;
+        uint64_t Q = std::max<uint64_t>((1ull << 27) / (RB * RB), 1);
+        pair_job_geometry(p, R, (uint32_t)Q);
+        r.nbins = p.nbins;
+        r.total_items = p.items;
+        r.units = N * (N - 1) / 2;
+    } else if (j.kind == CR_JOB_HAMMING_MC) {
+        if (j.k < 1 || j.k > 64) return fail(CR_ERR_INVALID_ARGUMENT, "Hamming request k=%u outside [1, 64]", j.k);
+        HamJob &h = r.hj;
+        h = HamJob{};
+        const uint32_t rbits = hamming_masks(j.k, h.masks);
+        h.source = kHamMc;
+        h.k = j.k;
+        h.first = 0;
+        h.count = j.samples;
+        h.seed = j.seed;
+        h.stream = j.stream;
+        h.nbins = j.k + rbits + 1;
+        r.nbins = h.nbins;
+        r.total_items = hamming_items(j.samples);
+        r.units = j.samples;
+    } else {
+        return fail(CR_ERR_INVALID_ARGUMENT, "unknown request kind %u", j.kind);
+    }
+    r.units_per_item = r.total_items ? (long double)r.units / r.total_items : 1;
+    r.footprint = ((uint64_t)kSlotBins * sizeof(uint64_t) + 255) / 256 * 256;
+    return CR_OK;
+}
+
+}  // namespace
+
+cr_status cr_server::iterate(std::vector<Request *> &run) {
+    // slices: equal share of the iteration's device time per request
+    const long double budget_ms = cfg.iteration_ms;
+    const long double share = budget_ms / std::max<size_t>(run.size(), 1);
+    std::vector<PairJob> pjobs;
+    std::vector<HamJob> hjobs;
+    std::vector<std::pair<Request *, uint64_t>> took;
+    uint64_t ptotal = 0, htotal = 0;
+    uint32_t lds_pair = 2, lds_ham = 2;
+    bool ham64 = false;
+    long double pair_units = 0, ham_units = 0;
+    for (Request *r : run) {
+        const long double rate = r->kind == CR_JOB_HAMMING_MC ? sample_rate : pair_rate;
+        uint64_t n = (uint64_t)(share * rate / std::max<long double>(r->units_per_item, 1));
+        n = std::max<uint64_t>(n, 1);
+        n = std::min<uint64_t>(n, r->total_items - r->cursor);
+        unsigned long long *out = slab.as<unsigned long long>() + (size_t)r->slot * kSlotBins;
+        if (r->kind == CR_JOB_HAMMING_MC) {
+            HamJob h = r->hj;
+            h.out = out;
+            h.item_begin = htotal;
+            h.items = n;
+            h.item_offset = r->cursor;
+            htotal += n;
+            lds_ham = std::max(lds_ham, h.nbins + 1);
+            ham64 = ham64 || h.k > 32;
+            hjobs.push_back(h);
+            ham_units += n * r->units_per_item;
+        } else {
+            PairJob p = r->pj;
+            p.out = out;
+            p.item_begin = ptotal;
+            p.items = n;
+            p.item_offset = r->cursor;
+            ptotal += n;
+            lds_pair = std::max(lds_pair, p.nbins + 1);
+            pjobs.push_back(p);
+            pair_units += n * r->units_per_item;
+        }
+        took.push_back({r, n});
+    }
+    // a Hamming launch handles either 32-bit or 64-bit data words
+    std::vector<HamJob> h32, h64;
+    for (auto &h : hjobs) (h.k > 32 ? h64 : h32).push_back(h);
+    auto rebase = [](std::vector<HamJob> &v) {
+        uint64_t t = 0;
+        for (auto &h : v) {
+            h.item_begin = t;
+            t += h.items;
+        }
+        return t;
+    };
+    const uint64_t t32 = rebase(h32), t64 = rebase(h64);
+    const size_t bytes = pjobs.size() * sizeof(PairJob) + (h32.size() + h64.size()) * sizeof(HamJob);
+    CR_TRY(staging.reserve(bytes));
+    CR_TRY(pair_jobs.reserve(std::max<size_t>(pjobs.size(), 1) * sizeof(PairJob)));
+    CR_TRY(ham_jobs.reserve(std::max<size_t>(h32.size() + h64.size(), 1) * sizeof(HamJob)));
+    char *st8 = staging.as<char>();
+    memcpy(st8, pjobs.data(), pjobs.size() * sizeof(PairJob));
+    memcpy(st8 + pjobs.size() * sizeof(PairJob), h32.data(), h32.size() * sizeof(HamJob));
+    memcpy(st8 + pjobs.size() * sizeof(PairJob) + h32.size() * sizeof(HamJob), h64.data(), h64.size() * sizeof(HamJob));
+    if (!pjobs.empty())
+        CR_HIP(hipMemcpyAsync(pair_jobs.ptr, st8, pjobs.size() * sizeof(PairJob), hipMemcpyHostToDevice, stream));
+    if (!h32.empty() || !h64.empty())
+        CR_HIP(hipMemcpyAsync(ham_jobs.ptr, st8 + pjobs.size() * sizeof(PairJob),
+                              (h32.size() + h64.size()) * sizeof(HamJob), hipMemcpyHostToDevice, stream));
+    CR_HIP(hipEventRecord(ev0, stream));
+    unsigned long long *ctr = counter.as<unsigned long long>();
+    if (!pjobs.empty()) {
+        PairLaunchConfig pc;
+        pc.R = cfg.rows_per_lane;
+        pc.lds_rows = lds_pair;
+        CR_TRY(launch_pairs(pair_jobs.as<PairJob>(), (uint32_t)pjobs.size(), ptotal, ctr, pc, ctx->cus, stream,
+                            nullptr, nullptr));
+    }
+    if (!h32.empty())
+        CR_TRY(launch_hamming(ham_jobs.as<HamJob>(), (uint32_t)h32.size(), t32, false, ctr + 1, lds_ham, ctx->cus,
+                              stream, nullptr, nullptr));
+    if (!h64.empty())
+        CR_TRY(launch_hamming(ham_jobs.as<HamJob>() + h32.size(), (uint32_t)h64.size(), t64, true, ctr + 2, lds_ham,
+                              ctx->cus, stream, nullptr, nullptr));
+    CR_HIP(hipEventRecord(ev1, stream));
+    CR_HIP(hipStreamSynchronize(stream));
+    float ms = 0;
+    CR_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    (void)ham64;
+    // refine the throughput model from this iteration (exponential average)
+    if (ms > 0.05f) {
+        const long double tot = pair_units / pair_rate + ham_units / sample_rate;
+        if (tot > 0) {
+            const long double scale = tot / ms;
+            pair_rate = 0.7L * pair_rate + 0.3L * pair_rate * scale;
+            sample_rate = 0.7L * sample_rate + 0.3L * sample_rate * scale;
+        }
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    st.iterations++;
+    st.busy_device_ms += ms;
+    for (auto &tk : took) {
+        tk.first->cursor += tk.second;
+        tk.first->iterations++;
+    }
+    return CR_OK;
+}
+
+void cr_server::loop() {
+    (void)hipSetDevice(ctx->device);
+    for (;;) {
+        std::vector<Request *> run;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            // retire cancelled active requests
+            for (auto it = active.begin(); it != active.end();) {
+                Request *r = reqs[*it].get();
+                if (r->cancel) {
+                    r->state = 2;
+                    r->status = CR_ERR_CANCELLED;
+                    r->t_finish = now();
+                    footprint -= r->footprint;
+                    free_slots.push_back(r->slot);
+                    st.cancelled++;
+                    it = active.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            // admission: FIFO while a slot is free and the footprint fits
+            while (!queue.empty()) {
+                Request *r = reqs[queue.front()].get();
+                if (r->cancel) {
+                    r->state = 2;
+                    r->status = CR_ERR_CANCELLED;
+                    r->t_finish = now();
+                    st.cancelled++;
+                    queue.pop_front();
+                    continue;
+                }
+                if (free_slots.empty() || footprint + r->footprint > cfg.hbm_budget_bytes) break;
+                queue.pop_front();
+                r->slot = free_slots.back();
+                free_slots.pop_back();
+                footprint += r->footprint;
+                r->state = 1;
+                r->t_admit = now();
+                active.push_back(r->id);
+                // zero the slot before the request's first pass (stream order)
+                (void)fill_zero_u64(slab.as<unsigned long long>() + (size_t)r->slot * kSlotBins, kSlotBins, stream);
+            }
+            cv_done.notify_all();
+            if (active.empty()) {
+                if (stop) return;
+                cv_work.wait(lk, [&] { return stop || !queue.empty(); });
+                continue;
+            }
+            for (uint64_t id : active) run.push_back(reqs[id].get());
+        }
+        cr_status s = iterate(run);
+        std::vector<Request *> fin;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (auto it = active.begin(); it != active.end();) {
+                Request *r = reqs[*it].get();
+                if (s != CR_OK || r->cursor >= r->total_items) {
+                    fin.push_back(r);
+                    it = active.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        }
+        for (Request *r : fin) {
+            std::vector<uint64_t> h(r->nbins, 0);
+            if (s == CR_OK) {
+                hipError_t e = hipMemcpyAsync(readback.ptr, slab.as<unsigned long long>() + (size_t)r->slot * kSlotBins,
+                                              r->nbins * sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(stream);
+                if (e == hipSuccess)
+                    memcpy(h.data(), readback.ptr, r->nbins * sizeof(uint64_t));
+                else
+                    s = fail(CR_ERR_DEVICE, "result copy failed: %s", hipGetErrorString(e));
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            r->result.swap(h);
+            r->status = s;
+            r->state = 2;
+            r->t_finish = now();
+            footprint -= r->footprint;
+            free_slots.push_back(r->slot);
+            st.completed++;
+        }
+        cv_done.notify_all();
+    }
+}
+
+extern "C" {
+
+cr_status cr_server_create(cr_context *ctx, const cr_server_config *cfg, cr_server **out) {
+    clear_error();
+    if (!ctx || !out) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    CR_HIP(hipSetDevice(ctx->device));
+    auto s = std::make_unique<cr_server>();
+    s->ctx = ctx;
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.hbm_budget_bytes == 0) s->cfg.hbm_budget_bytes = 1ull << 30;
+    if (s->cfg.max_active == 0) s->cfg.max_active = 64;
+    if (s->cfg.rows_per_lane == 0) s->cfg.rows_per_lane = 8;
+    if (s->cfg.iteration_ms <= 0) s->cfg.iteration_ms = 50;
+    if (pair_waves_per_cu(s->cfg.rows_per_lane, 66) == 0)
+        return fail(CR_ERR_INVALID_ARGUMENT, "rows_per_lane must be 2, 4, 8 or 16");
+    CR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    CR_HIP(hipEventCreate(&s->ev0));
+    CR_HIP(hipEventCreate(&s->ev1));
+    CR_TRY(s->slab.reserve((size_t)s->cfg.max_active * kSlotBins * sizeof(uint64_t)));
+    CR_TRY(s->counter.reserve(4 * sizeof(uint64_t)));
+    CR_TRY(s->readback.reserve(kSlotBins * sizeof(uint64_t)));
+    for (int i = (int)s->cfg.max_active - 1; i >= 0; --i) s->free_slots.push_back(i);
+    s->t0 = std::chrono::steady_clock::now();
+    cr_server *raw = s.release();
+    raw->worker = std::thread([raw] { raw->loop(); });
+    *out = raw;
+    return CR_OK;
+}
+
+cr_status cr_server_submit(cr_server *s, const cr_job *job, uint64_t *id) {
+    clear_error();
+    if (!s || !job || !id) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    auto r = std::make_unique<Request>();
+    CR_TRY(describe(s, *job, *r));
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (r->footprint > s->cfg.hbm_budget_bytes) {
+        s->st.rejected++;
+        return fail(CR_ERR_ADMISSION, "request needs %llu device bytes, budget is %llu",
+                    (unsigned long long)r->footprint, (unsigned long long)s->cfg.hbm_budget_bytes);
+    }
+    if (s->stop) return fail(CR_ERR_CANCELLED, "server is shutting down");
+    r->id = s->next_id++;
+    r->t_submit = s->now();
+    *id = r->id;
+    s->queue.push_back(r->id);
+    s->reqs[r->id] = std::move(r);
+    s->st.submitted++;
+    s->cv_work.notify_one();
+    return CR_OK;
+}
+
+cr_status cr_server_result(cr_server *s, uint64_t id, uint64_t *hist, uint32_t nbins, cr_job_result *res) {
+    clear_error();
+    if (!s) return fail(CR_ERR_INVALID_ARGUMENT, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(CR_ERR_NOT_FOUND, "unknown request %llu", (unsigned long long)id);
+    Request *r = it->second.get();
+    if (res) {
+        memset(res, 0, sizeof(*res));
+        res->done = r->state == 2;
+        res->status = r->status;
+        res->nbins = r->nbins;
+        res->iterations = r->iterations;
+        res->submit_s = r->t_submit;
+        res->admit_s = r->t_admit;
+        res->finish_s = r->t_finish;
+        res->units = r->units;
+    }
+    if (r->state != 2) return fail(CR_ERR_BUSY, "request %llu not finished", (unsigned long long)id);
+    if (r->status != CR_OK) return fail((cr_status)r->status, "request %llu failed", (unsigned long long)id);
+    if (hist) {
+        if (nbins < r->nbins) return fail(CR_ERR_INVALID_ARGUMENT, "nbins=%u < %u", nbins, r->nbins);
+        memset(hist, 0, nbins * sizeof(uint64_t));
+        memcpy(hist, r->result.data(), r->nbins * sizeof(uint64_t));
+    }
+    return CR_OK;
+}
+
+cr_status cr_server_wait(cr_server *s, uint64_t id, double timeout_ms) {
+    clear_error();
+    if (!s) return fail(CR_ERR_INVALID_ARGUMENT, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(CR_ERR_NOT_FOUND, "unknown request %llu", (unsigned long long)id);
+    Request *r = it->second.get();
+    auto pred = [&] { return r->state == 2; };
+    if (timeout_ms < 0) {
+        s->cv_done.wait(lk, pred);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), pred)) {
+        return fail(CR_ERR_BUSY, "request %llu still running", (unsigned long long)id);
+    }
+    return CR_OK;
+}
+
+cr_status cr_server_cancel(cr_server *s, uint64_t id) {
+    clear_error();
+    if (!s) return fail(CR_ERR_INVALID_ARGUMENT, "NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(CR_ERR_NOT_FOUND, "unknown request %llu", (unsigned long long)id);
+    if (it->second->state != 2) it->second->cancel = true;
+    s->cv_work.notify_one();
+    return CR_OK;
+}
+
+cr_status cr_server_stats_get(cr_server *s, cr_server_stats *out) {
+    clear_error();
+    if (!s || !out) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->st;
+    out->active = s->active.size();
+    out->queued = s->queue.size();
+    out->footprint_bytes = s->footprint;
+    return CR_OK;
+}
+
+double cr_server_now(cr_server *s) { return s ? s->now() : 0.0; }
+
+cr_status cr_server_destroy(cr_server *s) {
+    if (!s) return CR_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        for (auto &kv : s->reqs)
+            if (kv.second->state != 2) kv.second->cancel = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipSetDevice(s->ctx->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    s->slab.release();
+    s->pair_jobs.release();
+    s->ham_jobs.release();
+    s->counter.release();
+    s->staging.release();
+    s->readback.release();
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+    return CR_OK;
+}
+
+}  // extern "C"

@@ -76,11 +76,191 @@ cr_status cr_an_pairs(cr_context *ctx, uint64_t A, uint64_t num_words, uint64_t 
 /* Tuning knobs for the pair kernel; 0 = automatic (autotuned table). */
 typedef struct cr_pair_tuning {
     uint32_t rows_per_lane;   /* codewords held in registers per lane: 2, 4, 8 or 16 */
-    uint32_t chunk_blocks;    /* column chunk in units of the 64*rows row block */
+    uint32_t chunk_blocks;    /* column chunk in units of the 64*rows row block (single code) */
     uint32_t waves;           /* persistent waves launched (0 = fill the device) */
+    uint32_t items_per_wave;  /* batch: target work items per resident wave (0 = 16) */
 } cr_pair_tuning;
 cr_status cr_an_pairs_tuned(cr_context *ctx, uint64_t A, uint64_t num_words, uint64_t *raw, uint32_t nbins,
                             const cr_pair_tuning *tuning, double *kernel_ms);
+
+/* --- reference grid approximation ------------------------------------------
+ * Kernel-level histogram of run_ancoding_grid
+ * (distance_distribution/src/an_coding_grid.cu:215-322, one device): after the
+ * reference normalisation (A a power of two -> 1; m1, m2 clamped to 2^n), the
+ * sample list v_k = A*(T)(k * 2^n/m1) produced by the reference kernel loop
+ * (1D: an_coding_grid.cu:60-107 incl. its unrolled overshoot; 2D: :121-140),
+ * T = uint32 or uint64 as the reference dispatches, is compared with
+ *   gdim 1: every code word A*i, i < 2^n (ordered pairs, self pairs included)
+ *   gdim 2: the grid words A*(T)(i * 2^n/m2), i < m2 (bin 0 not counted).
+ * hist[b] gets the raw pair count at distance b (b < nbins; nbins >= 65 is
+ * always enough).  Scale with cr_an_grid_report.  m1 >= 3 for gdim 1. */
+cr_status cr_an_grid_ref(cr_context *ctx, int gdim, uint32_t n, uint64_t m1, uint64_t m2, uint64_t A, uint64_t *hist,
+                         uint32_t nbins, double *kernel_ms);
+
+/* Host scaling of run_ancoding_grid (an_coding_grid.cu:316-322):
+ * out[i] = (uint128)((long double)factor * hist[i]), factor = 2^n/m1 (gdim 1) or
+ * 4^n/(m1*m2) (gdim 2), with m1, m2 clamped as above.  out holds count_counts
+ * little-endian 128-bit values as (lo, hi) uint64 pairs. */
+cr_status cr_an_grid_report(int gdim, uint32_t n, uint64_t m1, uint64_t m2, const uint64_t *hist,
+                            uint32_t count_counts, uint64_t *out_u128);
+
+/* --- Extended Hamming ---------------------------------------------------------
+ * Weight histogram of the non-extended codewords (popcount of data plus the
+ * parity bits popcount(d & mask_t) & 1, computeHamming<N>,
+ * distance_distribution/src/hamming.cu:39-59) over a stream of data words.
+ * The masks are the reference's (hamming.h:54-64), extended to 64-bit data by
+ * the patterns of legacy/Hamming/ExtHamming64.hpp:31-37, truncated to k bits:
+ * k=8 -> (13,8), 16 -> (22,16), 24 -> (30,24), 32 -> (39,32), 64 -> (72,64). */
+typedef enum cr_ham_source {
+    CR_HAM_RANGE = 0,  /* d = first + s, s < count (exhaustive when first=0, count=2^k) */
+    CR_HAM_STRIDE = 1, /* d = (first + s*stride) mod 2^k */
+    CR_HAM_LIST = 2,   /* d = list[s] (host array of count words) */
+    CR_HAM_MC = 3      /* d = Monte Carlo word (first + s) of the Philox stream (seed, stream) */
+} cr_ham_source;
+
+typedef struct cr_ham_request {
+    uint32_t k;         /* data bits, 1..64 */
+    uint32_t source;    /* cr_ham_source */
+    uint64_t first;
+    uint64_t count;
+    uint64_t stride;
+    const uint64_t *list;
+    uint64_t seed;
+    uint32_t stream;
+    uint32_t reserved;
+} cr_ham_request;
+
+/* Number of parity bits (without the overall parity bit) for k data bits. */
+uint32_t cr_hamming_parity_bits(uint32_t k);
+
+/* hist[w] = samples whose non-extended codeword has weight w, w <= k + parity
+ * bits (nbins must exceed that).  Monte Carlo words: sample x of the stream is
+ * word x%4 (k <= 32; masked to k bits) or the word pair 2(x%2), 2(x%2)+1
+ * (k > 32) of Philox4x32-10 block x/4 (x/2) with counter
+ * (block, block >> 32, stream, 0x43520001) and key (seed, seed >> 32);
+ * `first` must be a multiple of 4 (k <= 32) or 2.  Replaces dhamming,
+ * dhamming_grid_1d and dhamming_mc (hamming.cu:64-81, hamming_grid.cu:53-74,
+ * hamming_mc.cu:57-81). */
+cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *req, uint64_t *hist, uint32_t nbins,
+                             double *kernel_ms);
+
+/* Post-processing of run_hamming (hamming.cu:182-208) from a non-extended
+ * weight histogram: out[0] = 2^k, out[1] = N*2^k with spheres, odd weights
+ * merged into the next even bin, the 1-bit-sphere closed form for odd bins,
+ * bins >= 3 shifted by k.  N = k + parity bits + 1; out has N+1 uint128 values
+ * as (lo, hi) pairs. */
+cr_status cr_hamming_report(uint32_t k, const uint64_t *weights, int with_1bit, uint64_t *out_u128);
+
+/* --- AN Monte Carlo -------------------------------------------------------------
+ * hist[b] = #{(i, j) : i < 2^n, j < m, popcount(A*i XOR A*u_ij) == b} with u_ij
+ * word j of the Philox stream (seed, i) masked to n bits (counter
+ * (j/4, 0, i, 0x43520001), key (seed, seed >> 32), word j%4).  Replaces
+ * dancoding_mc_shared's cuRAND Sobol draw (an_coding_mc.cu:94-158), which is
+ * not reproducible; n <= 32. */
+cr_status cr_an_mc(cr_context *ctx, uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint64_t *hist, uint32_t nbins,
+                   double *kernel_ms);
+
+/* --- batched exhaustive AN ---------------------------------------------------------
+ * Many codes in one persistent launch.  Code c is (A_c, num_words_c); its raw
+ * histogram (as cr_an_pairs) is written to raw[c*stride .. c*stride+stride).
+ * Work items of all codes share one device queue ordered longest code first,
+ * so codes of very different size (k = 8 .. 24) balance across the device.
+ * Replaces the one-launch-per-code loops of the super-A searches
+ * (distance_distribution_super_a/src/main.cpp:194-297, which call
+ * super_a/src/an_coding.cu:135-244 once per A). */
+typedef struct cr_an_code {
+    uint64_t A;
+    uint64_t num_words;
+} cr_an_code;
+
+cr_status cr_an_batch(cr_context *ctx, const cr_an_code *codes, uint32_t ncodes, uint64_t *raw, uint32_t stride,
+                      const cr_pair_tuning *tuning, double *kernel_ms);
+
+/* A batch plan uploads the work list once and captures zero-fill, the pair
+ * kernel and the copy-back in a hipGraph; cr_batch_plan_run replays it.  The
+ * plan owns its device memory until destroyed.  Plans of one context must not
+ * run concurrently. */
+typedef struct cr_batch_plan cr_batch_plan;
+cr_status cr_batch_plan_create(cr_context *ctx, const cr_an_code *codes, uint32_t ncodes, uint32_t stride,
+                               const cr_pair_tuning *tuning, cr_batch_plan **out);
+cr_status cr_batch_plan_run(cr_batch_plan *plan, uint64_t *raw, double *elapsed_ms);
+cr_status cr_batch_plan_destroy(cr_batch_plan *plan);
+
+/* Autotuner: times candidate (rows per lane, items per wave) settings of the
+ * batched pair kernel on the given codes (each candidate runs the whole batch
+ * `repeats` times) and returns the fastest in *best. */
+cr_status cr_autotune_batch(cr_context *ctx, const cr_an_code *codes, uint32_t ncodes, uint32_t repeats,
+                            cr_pair_tuning *best, double *best_ms);
+
+/* --- serving: continuous batching ---------------------------------------------------
+ * A request loop that keeps histogram state on the device.  A scheduler
+ * thread runs iterations: it admits queued requests while their device
+ * footprint fits the HBM budget and a slot is free, then launches one pass of
+ * the pair kernel over a slice of every active AN request (exhaustive and
+ * integer-grid) and one pass of the Hamming kernel over a slice of every
+ * active Monte Carlo request, so requests that arrive while others run join at
+ * the next iteration and their work shares the device.  Results are
+ * independent of the mix: integer histograms are order-independent sums and
+ * Monte Carlo words come from fixed Philox counters.  No reference
+ * counterpart (the reference runs one code per process call). */
+typedef enum cr_job_kind {
+    CR_JOB_AN_EXHAUSTIVE = 0, /* raw histogram of the code A*d, d < 2^k (as cr_an_pairs) */
+    CR_JOB_AN_GRID = 1,       /* integer grid: words A*(i*stride), i*stride < 2^k, unordered pairs (as cr_an_pairs
+                                 with multiplier A*stride and ceil(2^k/stride) words) */
+    CR_JOB_HAMMING_MC = 2     /* Extended Hamming weights of `samples` Philox words (as cr_hamming_weights, CR_HAM_MC) */
+} cr_job_kind;
+
+typedef struct cr_job {
+    uint32_t kind;    /* cr_job_kind */
+    uint32_t k;       /* data bits */
+    uint64_t A;       /* AN jobs */
+    uint64_t stride;  /* AN grid */
+    uint64_t samples; /* Hamming Monte Carlo */
+    uint64_t seed;
+    uint32_t stream;
+    uint32_t reserved;
+} cr_job;
+
+typedef struct cr_server_config {
+    uint64_t hbm_budget_bytes; /* sum of device footprints of active requests (0 = 1 GiB) */
+    uint32_t max_active;       /* active request slots (0 = 64) */
+    uint32_t rows_per_lane;    /* pair kernel R (0 = 8) */
+    double iteration_ms;       /* target device time of one iteration (0 = 50 ms) */
+} cr_server_config;
+
+typedef struct cr_job_result {
+    uint32_t done;            /* 1 when finished */
+    uint32_t status;          /* cr_status of the request */
+    uint32_t nbins;           /* bins written */
+    uint32_t iterations;      /* iterations the request took part in */
+    double submit_s, admit_s, finish_s; /* seconds on the server's clock */
+    uint64_t units;           /* pairs (AN) or samples (Hamming) */
+} cr_job_result;
+
+typedef struct cr_server_stats {
+    uint64_t submitted, completed, cancelled, rejected;
+    uint64_t iterations;
+    uint64_t active, queued;
+    uint64_t footprint_bytes;  /* device bytes held by active requests */
+    double busy_device_ms;     /* summed device time of the iterations */
+} cr_server_stats;
+
+typedef struct cr_server cr_server;
+cr_status cr_server_create(cr_context *ctx, const cr_server_config *cfg, cr_server **out);
+/* Queues a request; *id identifies it.  Fails with CR_ERR_ADMISSION if the
+ * request alone exceeds the HBM budget, CR_ERR_INVALID_ARGUMENT if malformed. */
+cr_status cr_server_submit(cr_server *s, const cr_job *job, uint64_t *id);
+/* Copies the result of a finished request (CR_ERR_BUSY while it runs).  hist
+ * may be NULL to query the state only. */
+cr_status cr_server_result(cr_server *s, uint64_t id, uint64_t *hist, uint32_t nbins, cr_job_result *res);
+/* Blocks until the request finished (or timeout_ms elapsed: CR_ERR_BUSY). */
+cr_status cr_server_wait(cr_server *s, uint64_t id, double timeout_ms);
+cr_status cr_server_cancel(cr_server *s, uint64_t id);
+cr_status cr_server_stats_get(cr_server *s, cr_server_stats *out);
+/* Seconds on the server's clock (the clock of cr_job_result times). */
+double cr_server_now(cr_server *s);
+/* Stops the scheduler (cancelling unfinished requests) and frees the server. */
+cr_status cr_server_destroy(cr_server *s);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,134 @@
+"""GPU parity of the Extended Hamming weight kernel (all sources) and of the AN
+Monte Carlo kernel: bit-exact against the oracle on shared inputs / Philox
+counters, exact against the golden vectors and the MacWilliams closed form,
+and within 4-sigma binomial bounds at full Monte Carlo sizes."""
+import math
+
+import numpy as np
+import pytest
+
+import coding_reliability_amd as cra
+from conftest import macwilliams_weights
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cra.Context(0)
+    yield c
+    c.close()
+
+
+def merged_even(h):
+    """Extended-codeword weights from non-extended weights (odd -> next even)."""
+    n = len(h) + 1
+    return [(h[w] + (h[w - 1] if w else 0)) if w % 2 == 0 and w < len(h) else 0 for w in range(n)]
+
+
+@pytest.mark.parametrize("k", [4, 8, 12, 16, 20, 24])
+def test_exhaustive_vs_oracle(ctx, k):
+    h, _ = ctx.hamming_weights(k)
+    assert h == O.hamming_weights(k)[: len(h)]
+
+
+@pytest.mark.parametrize("k,name", [(8, "sol_hamming_n8"), (16, "sol_hamming_n16"), (24, "sol_hamming_n24"),
+                                    (32, "sol_hamming_n32")])
+def test_exhaustive_golden(ctx, solutions, k, name):
+    # run_hamming's reported even bins equal solutions.h << k (hamming.cu:183-208)
+    h, ms = ctx.hamming_weights(k)
+    assert sum(h) == 1 << k
+    rep = cra.hamming_report(k, h, True)
+    sol = solutions[name]
+    assert all(rep[i] == sol[i] << k for i in range(4, len(sol), 2))
+
+
+def test_exhaustive_k32_macwilliams(ctx):
+    h, _ = ctx.hamming_weights(32)
+    E = merged_even(h)
+    assert E[: 40] == macwilliams_weights(32)[: 40]
+
+
+def test_range_subrange_and_k64_window(ctx):
+    # a window of the (72,64) code: d in [2^63 - 2^20, 2^63 + 2^20)
+    k, first, count = 64, (1 << 63) - (1 << 20), 1 << 21
+    h, _ = ctx.hamming_weights(k, cra.HAM_RANGE, first=first, count=count)
+    words = np.arange(count, dtype=np.uint64) + np.uint64(first)
+    ref = O.hamming_weights_list(k, words)
+    assert h == ref[: len(h)]
+
+
+def test_stride_grid_c3(ctx):
+    # (39,32) integer grid d_i = i*65537 mod 2^32, i < 2^20
+    k, stride, m = 32, 65537, 1 << 20
+    h, _ = ctx.hamming_weights(k, cra.HAM_STRIDE, first=0, count=m, stride=stride)
+    words = (np.arange(m, dtype=np.uint64) * np.uint64(stride)) & np.uint64(0xFFFFFFFF)
+    assert h == O.hamming_weights_list(k, words)[: len(h)]
+    assert sum(h) == m
+
+
+def test_list_reference_grid_samples(ctx):
+    # the sample points of the reference grid kernel (hamming_grid.cu:53-74), k=32, m=2^20
+    k, m, S = 32, 1 << 20, 256
+    step = 1.0 * (1 << k) / m
+    words = O.hamming_grid_ref_samples(m, step, S)
+    h, _ = ctx.hamming_weights(k, cra.HAM_LIST, words=words)
+    assert h == O.hamming_weights_list(k, words)[: len(h)]
+
+
+@pytest.mark.parametrize("k,first,count", [(32, 0, 1 << 16), (16, 4096, 100003), (64, 0, 1 << 15), (64, 2, 33333),
+                                           (57, 10, 4096)])
+def test_mc_bitexact_vs_oracle(ctx, k, first, count):
+    h, _ = ctx.hamming_weights(k, cra.HAM_MC, first=first, count=count, seed=0xC0FFEE, stream=7)
+    assert h == O.hamming_mc(k, 0xC0FFEE, 7, first, first + count)[: len(h)]
+
+
+def test_mc_c3_2pow34_within_4sigma(ctx):
+    # C3 Monte Carlo: 2^34 Philox words of the (39,32) code vs the exhaustive truth
+    k, S = 32, 1 << 34
+    h, ms = ctx.hamming_weights(k, cra.HAM_MC, first=0, count=S, seed=20261202, stream=0)
+    truth, _ = ctx.hamming_weights(k)
+    print(f"MC 2^34 samples (39,32): {ms:.1f} ms", flush=True)
+    assert sum(h) == S
+    for w, t in enumerate(truth):
+        p = t / (1 << k)
+        assert abs(h[w] - S * p) <= 4 * math.sqrt(S * p * (1 - p)) + 1, w
+
+
+def test_mc_72_64_within_4sigma_macwilliams(ctx):
+    # (72,64): no exhaustive run is possible; truth from the MacWilliams identity
+    k, S = 64, 1 << 30
+    h, ms = ctx.hamming_weights(k, cra.HAM_MC, first=0, count=S, seed=99, stream=3)
+    A = macwilliams_weights(64)
+    E = merged_even(h)
+    print(f"MC 2^30 samples (72,64): {ms:.1f} ms", flush=True)
+    for w in range(0, 73, 2):
+        p = A[w] / 2.0 ** 64
+        assert abs(E[w] - S * p) <= 4 * math.sqrt(S * p * (1 - p)) + 1, w
+
+
+@pytest.mark.parametrize("A,n,m", [(61, 8, 16), (641, 10, 100), (58659, 12, 64), (3, 16, 5)])
+def test_an_mc_bitexact_vs_oracle(ctx, A, n, m):
+    h, _ = ctx.an_mc(A, n, m, seed=1234)
+    assert h == O.an_mc(A, n, m, 1234)[: len(h)]
+
+
+def test_an_mc_4sigma_vs_exhaustive(ctx):
+    A, n, m = 58659, 16, 4096
+    h, _ = ctx.an_mc(A, n, m, seed=77)
+    raw, _ = ctx.an_pairs(A, 1 << n)
+    tot = (1 << n) * m
+    assert sum(h) == tot
+    for b in range(len(raw)):
+        p = (2 * raw[b] + ((1 << n) if b == 0 else 0)) / 4 ** n
+        assert abs(h[b] - tot * p) <= 4 * math.sqrt(tot * p * (1 - p)) + 1, b
+
+
+def test_invalid(ctx):
+    with pytest.raises(cra.CRError):
+        ctx.hamming_weights(65)
+    with pytest.raises(cra.CRError):
+        ctx.hamming_weights(32, cra.HAM_MC, first=1, count=10)
+    with pytest.raises(cra.CRError):
+        ctx.an_mc(61, 33, 10, 1)

@@ -1,0 +1,116 @@
+"""GPU tests of the continuous-batching request loop: every result is the same
+alone and in a mix (bit-exact; Monte Carlo through fixed Philox counters),
+admission by HBM footprint, refusal, cancellation."""
+import time
+
+import pytest
+
+import coding_reliability_amd as cra
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cra.Context(0)
+    yield c
+    c.close()
+
+
+MIX = [
+    dict(kind=cra.JOB_AN_EXHAUSTIVE, k=18, A=58659),
+    dict(kind=cra.JOB_AN_GRID, k=32, A=61, stride=65537 * 16),
+    dict(kind=cra.JOB_HAMMING_MC, k=64, samples=1 << 24, seed=5, stream=1),
+    dict(kind=cra.JOB_AN_EXHAUSTIVE, k=16, A=641),
+    dict(kind=cra.JOB_HAMMING_MC, k=32, samples=(1 << 22) + 12, seed=9, stream=2),
+    dict(kind=cra.JOB_AN_EXHAUSTIVE, k=20, A=32771),
+    dict(kind=cra.JOB_AN_GRID, k=24, A=12345, stride=4097),
+]
+
+
+def direct(ctx, j):
+    """The same request through the one-shot entry points."""
+    if j["kind"] == cra.JOB_AN_EXHAUSTIVE:
+        return ctx.an_pairs(j["A"], 1 << j["k"])[0]
+    if j["kind"] == cra.JOB_AN_GRID:
+        s = j["stride"]
+        return ctx.an_pairs(j["A"] * s, ((1 << j["k"]) + s - 1) // s)[0]
+    return ctx.hamming_weights(j["k"], cra.HAM_MC, first=0, count=j["samples"], seed=j["seed"],
+                               stream=j["stream"])[0]
+
+
+def test_alone_equals_mix(ctx):
+    ref = [direct(ctx, j) for j in MIX]
+    # alone: one request at a time
+    with cra.Server(ctx, iteration_ms=5) as srv:
+        alone = []
+        for j in MIX:
+            rid = srv.submit(**j)
+            srv.wait(rid)
+            alone.append(srv.result(rid)[0])
+    # mix: all at once, short iterations so the requests interleave
+    with cra.Server(ctx, iteration_ms=2) as srv:
+        ids = [srv.submit(**j) for j in MIX]
+        for rid in ids:
+            srv.wait(rid)
+        mixed = [srv.result(rid) for rid in ids]
+        st = srv.stats()
+    for j, r, a, (m, info) in zip(MIX, ref, alone, mixed):
+        assert a == r[: len(a)], j
+        assert m == r[: len(m)], j
+    assert max(info["iterations"] for _, info in mixed) > 1
+    assert st["completed"] == len(MIX)
+
+
+def test_exhaustive_request_vs_oracle(ctx):
+    with cra.Server(ctx) as srv:
+        rid = srv.submit(cra.JOB_AN_EXHAUSTIVE, 12, A=58659)
+        srv.wait(rid)
+        h, info = srv.result(rid)
+    assert h == O.an_pairs_raw(58659, 1 << 12, len(h))
+    assert info["units"] == (1 << 12) * ((1 << 12) - 1) // 2
+
+
+def test_admission_by_footprint(ctx):
+    # budget for exactly one histogram slot: requests run one after another
+    with cra.Server(ctx, hbm_budget_bytes=1280, iteration_ms=2) as srv:
+        ids = [srv.submit(cra.JOB_AN_EXHAUSTIVE, 16, A=A) for A in (61, 641, 58659)]
+        for rid in ids:
+            srv.wait(rid)
+        infos = [srv.result(rid)[1] for rid in ids]
+    # each admitted only after the previous one finished
+    for a, b in zip(infos, infos[1:]):
+        assert b["admit_s"] >= a["finish_s"] - 1e-6
+
+
+def test_rejected_when_over_budget(ctx):
+    with cra.Server(ctx, hbm_budget_bytes=64) as srv:
+        with pytest.raises(cra.CRError) as e:
+            srv.submit(cra.JOB_AN_EXHAUSTIVE, 10, A=61)
+        assert e.value.status == 6
+        assert srv.stats()["rejected"] == 1
+
+
+def test_cancel(ctx):
+    with cra.Server(ctx, iteration_ms=5) as srv:
+        big = srv.submit(cra.JOB_AN_EXHAUSTIVE, 26, A=58659)  # minutes of work
+        small = srv.submit(cra.JOB_AN_EXHAUSTIVE, 14, A=61)
+        srv.wait(small, 60000)
+        srv.cancel(big)
+        srv.wait(big, 60000)
+        with pytest.raises(cra.CRError) as e:
+            srv.result(big)
+        assert e.value.status == 7
+        h, _ = srv.result(small)
+        assert h == O.an_pairs_raw(61, 1 << 14, len(h))
+
+
+def test_invalid_requests(ctx):
+    with cra.Server(ctx) as srv:
+        with pytest.raises(cra.CRError):
+            srv.submit(7, 10)
+        with pytest.raises(cra.CRError):
+            srv.submit(cra.JOB_AN_EXHAUSTIVE, 0, A=3)
+        with pytest.raises(cra.CRError):
+            srv.result(12345)
