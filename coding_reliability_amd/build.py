@@ -60,6 +60,26 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+REFERENCE = os.environ.get("CR_REFERENCE_ROOT", "/root/reference")
+REF_CLI = os.path.join(BUILD, "codrel_ref")
+
+
+def build_reference_cli() -> str | None:
+    """Compiles the reference's unchanged distance_distribution/src/main.cpp
+    (with its own globals.h / algorithms.h) and links it against libcodrel.so,
+    which provides every entry point it calls.  Only where the reference
+    sources are mounted; the binary lands in _build/."""
+    src_dir = os.path.join(REFERENCE, "distance_distribution", "src")
+    main = os.path.join(src_dir, "main.cpp")
+    if not os.path.exists(main):
+        return None
+    os.makedirs(BUILD, exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O2", "-w", "-I", src_dir, main, "-L", PKG, "-lcodrel",
+           "-Wl,-rpath,$ORIGIN/../coding_reliability_amd", "-o", REF_CLI]
+    subprocess.run(cmd, check=True)
+    return REF_CLI
+
+
 def dump_isa(src_name: str, out_dir: str) -> str:
     """Writes the gfx950 assembly of one source file (for inspection)."""
     os.makedirs(out_dir, exist_ok=True)
@@ -72,5 +92,6 @@ def dump_isa(src_name: str, out_dir: str) -> str:
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_reference_cli()
     if len(sys.argv) > 2 and sys.argv[1] == "--isa":
         print(dump_isa(sys.argv[2], os.path.join(ROOT, "_work", "isa")))
