@@ -11,7 +11,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (CRError, check, cr_an_code, cr_device_info, cr_ham_request, cr_job, cr_job_result,
-                   cr_pair_tuning, cr_server_config, cr_server_stats, lib)
+                   cr_pair_tuning, cr_server_config, cr_server_stats, cr_supera_request, lib)
 
 JOB_AN_EXHAUSTIVE, JOB_AN_GRID, JOB_HAMMING_MC = 0, 1, 2
 
@@ -149,6 +149,46 @@ class Context:
         ms = C.c_double(0)
         check(lib().cr_autotune_batch(self._h, self._codes(codes), len(codes), repeats, C.byref(best), C.byref(ms)))
         return dict(rows_per_lane=best.rows_per_lane, items_per_wave=best.items_per_wave), ms.value
+
+    def supera_search(self, style: int, k: int, h_bits: int = 0, A_start: int = 0, A_end: int = 0,
+                      As=None, with_grid: int = 0, m1: int = 0, m2: int = 0, verbose: int = 0) -> tuple[str, float]:
+        """Super-A search output text (style 0: distance_distribution stdout,
+        style 1: super_a CSV file body) and the device seconds it took."""
+        lst = np.ascontiguousarray(As if As is not None else [], dtype=np.uint64)
+        q = cr_supera_request(style, k, h_bits, with_grid, A_start, A_end,
+                              _u64p(lst) if len(lst) else None, len(lst), verbose, m1, m2)
+        txt = C.c_void_p()
+        secs = C.c_double(0)
+        check(lib().cr_supera_search(self._h, C.byref(q), C.byref(txt), C.byref(secs)))
+        try:
+            s = C.string_at(txt).decode()
+        finally:
+            lib().cr_free(txt)
+        return s, secs.value
+
+    def an_candidates(self, k: int, abits: int) -> tuple[int, list[int], float]:
+        """(best minimum NAF weight, sorted A attaining it, kernel ms) for one
+        (data width, A width) cell of ANcandidates."""
+        cap = 1 << 20
+        best = np.zeros(1, dtype=np.uint64)
+        n = np.zeros(1, dtype=np.uint64)
+        cands = np.zeros(cap, dtype=np.uint64)
+        ms = C.c_double(0)
+        check(lib().cr_an_candidates(self._h, k, abits, _u64p(best), _u64p(cands), cap, _u64p(n), C.byref(ms)))
+        return int(best[0]), [int(x) for x in cands[: min(int(n[0]), cap)]], ms.value
+
+    def ancandidates_tsv(self, dmin: int, dmax: int, amin: int, amax: int) -> str:
+        """Output of the ANcandidates CLI for -d dmin -D dmax -a amin -A amax
+        (ANcandidates.cpp:246-299), candidates sorted."""
+        lines = [f"# bit width data = [{dmin},{dmax}]", f"# bit width A = [{amin},{amax}]",
+                 "# n=|C|\tk=|D|\t|A|\t|SDR|\t|candidates|\tcandidates"]
+        nd, na = dmax - dmin + 1, amax - amin + 1
+        for cell in range(nd * na):
+            k = cell % nd + dmin
+            a = cell // nd + amin
+            best, cands, _ = self.an_candidates(k, a)
+            lines.append(f"{k + a}\t{k}\t{a}\t{best}\t{len(cands)}\t" + ",".join(str(x) for x in cands))
+        return "\n".join(lines) + "\n"
 
     def hamming_weights(self, k: int, source: int = HAM_RANGE, first: int = 0, count: int | None = None,
                         stride: int = 0, words=None, seed: int = 0, stream: int = 0) -> tuple[list[int], float]:

@@ -66,6 +66,12 @@ class cr_server_stats(C.Structure):
                 ("queued", C.c_uint64), ("footprint_bytes", C.c_uint64), ("busy_device_ms", C.c_double)]
 
 
+class cr_supera_request(C.Structure):
+    _fields_ = [("style", C.c_uint32), ("k", C.c_uint32), ("h_bits", C.c_uint32), ("with_grid", C.c_int32),
+                ("A_start", C.c_uint64), ("A_end", C.c_uint64), ("As", C.POINTER(C.c_uint64)), ("nAs", C.c_uint32),
+                ("verbose", C.c_int32), ("m1", C.c_uint64), ("m2", C.c_uint64)]
+
+
 class cr_an_code(C.Structure):
     _fields_ = [("A", C.c_uint64), ("num_words", C.c_uint64)]
 
@@ -106,6 +112,11 @@ SIGNATURES = {
     "cr_server_stats_get": (C.c_int, [C.c_void_p, C.POINTER(cr_server_stats)]),
     "cr_server_now": (C.c_double, [C.c_void_p]),
     "cr_server_destroy": (C.c_int, [C.c_void_p]),
+    "cr_supera_search": (C.c_int, [C.c_void_p, C.POINTER(cr_supera_request), C.POINTER(C.c_void_p),
+                                   C.POINTER(C.c_double)]),
+    "cr_free": (None, [C.c_void_p]),
+    "cr_an_candidates": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, u64p, u64p, C.c_uint64, u64p,
+                                   C.POINTER(C.c_double)]),
     "cr_an_mc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, u64p, C.c_uint32,
                            C.POINTER(C.c_double)]),
 }

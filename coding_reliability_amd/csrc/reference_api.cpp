@@ -689,24 +689,26 @@ static std::string printbits_str(uintll_t v, uintll_t n) {  // super_a/src/globa
     return s;
 }
 
-void process_result(std::stringstream &ss, uintll_t A, uint_t h, const Flags &flags, double *times,
-                    uint128_t *counts) {
+namespace cr {
+
+// One CSV row of super_a's process_result with an explicit row counter.
+void supera_row(std::stringstream &ss, uintll_t A, uint_t h, const Flags &flags, double *times, uint128_t *counts,
+                unsigned *id) {
     // super_a/src/globals.cpp:39-116
-    static unsigned int id = 0;
     const char sep = ',';
     long double total = 0;
     const uint_t n = (uint_t)flags.n;
     for (uint_t i = 0; i < n + h + 1; ++i) total += static_cast<long double>(counts[i]);
     long double base1 = pow(2.0, n);
     const int cnts = (int)flags.n + (int)h + 1;
-    if (id == 0) {
+    if (*id == 0) {
         ss << "id, k, A, h, A_2, m_1, m_2, hlen";
         for (int i = 1; i <= cnts; ++i) ss << ", p_" << i;
         for (int i = 1; i <= cnts; ++i) ss << ", hist_" << i;
         ss << ", t_kernel, t, sum, minb, mincb, superA, minb2, mincb2, superA2\n";
     }
-    ss << id++ << sep << flags.n << sep << A << sep << h << sep << printbits_str(A, n) << sep << flags.mc_iterations
-       << sep << flags.mc_iterations_2 << sep << cnts;
+    ss << (*id)++ << sep << flags.n << sep << A << sep << h << sep << printbits_str(A, n) << sep
+       << flags.mc_iterations << sep << flags.mc_iterations_2 << sep << cnts;
     for (uint_t i = 1; i <= (uint_t)cnts; ++i) {
         const long double base2 = binom(static_cast<long double>(n + h), static_cast<long double>(i));
         const long double base = base1 * base2;
@@ -727,4 +729,12 @@ void process_result(std::stringstream &ss, uintll_t A, uint_t h, const Flags &fl
                   << flags.n << sep << A << sep << h << sep << flags.mc_iterations << sep << flags.mc_iterations_2
                   << "\n";
     }
+}
+
+}  // namespace cr
+
+void process_result(std::stringstream &ss, uintll_t A, uint_t h, const Flags &flags, double *times,
+                    uint128_t *counts) {
+    static unsigned int id = 0;
+    cr::supera_row(ss, A, h, flags, times, counts, &id);
 }

@@ -192,6 +192,44 @@ cr_status cr_batch_plan_destroy(cr_batch_plan *plan);
 cr_status cr_autotune_batch(cr_context *ctx, const cr_an_code *codes, uint32_t ncodes, uint32_t repeats,
                             cr_pair_tuning *best, double *best_ms);
 
+/* --- super-A search ------------------------------------------------------------------
+ * The search drivers over many A, with every exhaustive histogram computed in
+ * one batched launch (cr_an_batch) and the drivers' sequential bookkeeping
+ * replayed on the host in A order:
+ *   style 0: ancoding_search_super_A of distance_distribution/src/main.cpp:131-177
+ *            (-S h: odd A in (2^(h-1), 2^h); stdout lines "A: c[minb] = mincb"
+ *            for each new best, then the summary line; codes with k > 16 use
+ *            the reference grid with m1, m2 as the driver does);
+ *   style 1: ancoding_search_super_A(_by_file) of distance_distribution_super_a/src/main.cpp:194-297
+ *            (odd A in [A_start, A_end] or the list As; the CSV file body:
+ *            "Total Time [s]" line, device line, one process_result row per A
+ *            (super_a/src/globals.cpp:39-116) with the running-best columns).
+ * *text receives a malloc'ed NUL-terminated string (free with cr_free).
+ * Per-A times in the output are the batch time apportioned by pair count. */
+typedef struct cr_supera_request {
+    uint32_t style;     /* 0 or 1 */
+    uint32_t k;         /* data bits */
+    uint32_t h_bits;    /* style 0: the -S value */
+    int32_t with_grid;  /* style 1: 0 exhaustive, 1/2 reference grid; style 0: grid dimension for k > 16 */
+    uint64_t A_start, A_end;  /* style 1 range */
+    const uint64_t *As;       /* style 1 candidate list (used when nAs > 0) */
+    uint32_t nAs;
+    int32_t verbose;
+    uint64_t m1, m2;          /* grid sizes */
+} cr_supera_request;
+cr_status cr_supera_search(cr_context *ctx, const cr_supera_request *req, char **text, double *device_s);
+void cr_free(void *p);
+
+/* --- ANcandidates ----------------------------------------------------------------------
+ * For every odd A with exactly abits bits (2^(abits-1) < A < 2^abits) the
+ * minimum over d in [1, 2^k) of popcount(c XOR 3c), c = A*d (the NAF weight;
+ * worker, ANcandidates/ANcandidates.cpp:35-160); *best = the maximum of these
+ * minima, cands = the A attaining it in increasing order (the reference's
+ * order is nondeterministic; ANcandidates_check.cpp sorts before comparing),
+ * *ncands their number (at most cap written).  k + abits <= 62. */
+cr_status cr_an_candidates(cr_context *ctx, uint32_t k, uint32_t abits, uint64_t *best, uint64_t *cands,
+                           uint64_t cap, uint64_t *ncands, double *kernel_ms);
+
 /* --- serving: continuous batching ---------------------------------------------------
  * A request loop that keeps histogram state on the device.  A scheduler
  * thread runs iterations: it admits queued requests while their device
