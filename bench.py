@@ -12,7 +12,9 @@ Prints ONE JSON line (rank 0).  Extra keys beyond the required ones:
   roofline       VALU roofline of the pair kernel (events measured live)
   cpu_baseline   the oracle's restatement of the reference CPU loop on a
                  bounded row-shard sample of the same workload, host cores
-  batched        batched-vs-per-launch comparison (when --batched is given)
+  batched        batched (C4, one launch + captured hipGraph) vs one code per
+                 launch, and the super-A k=8 sweep (skip with --no-batched)
+  served         continuous-batching request mix (with --served)
 """
 from __future__ import annotations
 
@@ -72,6 +74,123 @@ def cpu_baseline(A: int, k: int, target_s: float) -> dict:
     }
 
 
+def run_batched(ctx, autotune: bool = True) -> dict:
+    """C4: 4096 AN codes (odd A uniform in [3, 2^16), k uniform in 8..18, seed
+    C4_SEED) -- one code per launch (cr_an_pairs per code: zero, kernel,
+    copy-back, sync) against one batched launch and its captured hipGraph;
+    plus the reference's published per-launch sweep shape (super-A, k=8,
+    16384 codes).  Every result is checked bit-exact against the per-code run."""
+    from coding_reliability_amd import workloads as WL
+    out = {}
+    codes = WL.c4_codes()
+    pairs = WL.pairs_of(codes)
+    tune = {}
+    if autotune:
+        best, ms = ctx.autotune_batch(codes[::16], repeats=1)
+        tune = best
+        log(f"autotune on 256 codes: {best} ({ms:.1f} ms)")
+    t0 = time.perf_counter()
+    per = [ctx.an_pairs(A, N)[0] for A, N in codes]
+    t_per = time.perf_counter() - t0
+    log(f"C4 per-launch: {t_per:.3f} s")
+    t0 = time.perf_counter()
+    got, kms = ctx.an_batch(codes, **tune)
+    t_batch = time.perf_counter() - t0
+    plan = ctx.batch_plan(codes, **tune)
+    plan.run()
+    reps, t_graph = 3, 0.0
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res, _ = plan.run()
+        t_graph += time.perf_counter() - t0
+        assert res == per, "graph replay differs from per-code launches"
+    t_graph /= reps
+    plan.close()
+    assert got == per, "batched launch differs from per-code launches"
+    log(f"C4 batched: {t_batch:.3f} s, graph replay: {t_graph:.3f} s")
+    out["c4"] = {
+        "workload": "4096 AN codes, odd A uniform in [3,2^16), k uniform in {8..18}, seed 20261202",
+        "pairs": pairs,
+        "per_launch_s": round(t_per, 4), "per_launch_gpairs": round(pairs / t_per / 1e9, 2),
+        "batched_s": round(t_batch, 4), "batched_gpairs": round(pairs / t_batch / 1e9, 2),
+        "batched_kernel_ms": round(kms, 3),
+        "graph_s": round(t_graph, 4), "graph_gpairs": round(pairs / t_graph / 1e9, 2),
+        "speedup_graph_vs_per_launch": round(t_per / t_graph, 3),
+        "tuning": tune, "bit_exact": True,
+    }
+    # the reference's published per-launch sweep: super-A k=8, 16384 codes
+    sweep = WL.supera_codes(8, 16)
+    spairs = WL.pairs_of(sweep)
+    t0 = time.perf_counter()
+    per8 = [ctx.an_pairs(A, N)[0] for A, N in sweep]
+    t_per8 = time.perf_counter() - t0
+    plan = ctx.batch_plan(sweep)
+    plan.run()
+    t0 = time.perf_counter()
+    res8, _ = plan.run()
+    t_g8 = time.perf_counter() - t0
+    plan.close()
+    assert res8 == per8
+    out["supera_k8"] = {
+        "workload": "super-A sweep k=8: all 16384 odd A in (2^15, 2^16)",
+        "per_launch_ms_per_code": round(t_per8 * 1e3 / len(sweep), 4),
+        "graph_ms_per_code": round(t_g8 * 1e3 / len(sweep), 5),
+        "speedup_graph_vs_per_launch": round(t_per8 / t_g8, 1),
+        "reference_per_launch_ms_per_code": 2.08,
+        "bit_exact": True,
+    }
+    log(f"super-A k=8: per-launch {t_per8:.3f} s, graph {t_g8 * 1e3:.2f} ms")
+    return out
+
+
+def run_served(ctx, njobs: int, rate: float, seed: int = 7) -> dict:
+    """C5: Poisson arrivals (rate jobs/s) of a uniform mix of exhaustive AN
+    (k in 18..24, odd 16-bit A), integer-grid AN (k=32, stride 4097, odd 16-bit
+    A) and Monte Carlo (72,64) Extended Hamming (2^30 samples) requests."""
+    import numpy as np
+    import coding_reliability_amd as cra
+    g = np.random.Generator(np.random.Philox(seed))
+    jobs, arrivals, t = [], [], 0.0
+    for i in range(njobs):
+        kind = int(g.integers(0, 3))
+        A = int(2 * g.integers(1 << 14, 1 << 15) + 1)
+        if kind == 0:
+            jobs.append(dict(kind=cra.JOB_AN_EXHAUSTIVE, k=int(g.integers(18, 25)), A=A))
+        elif kind == 1:
+            jobs.append(dict(kind=cra.JOB_AN_GRID, k=32, A=A, stride=4097))
+        else:
+            jobs.append(dict(kind=cra.JOB_HAMMING_MC, k=64, samples=1 << 30, seed=seed, stream=i))
+        t += float(g.exponential(1.0 / rate))
+        arrivals.append(t)
+    srv = cra.Server(ctx, iteration_ms=20)
+    t0 = srv.now()
+    ids = []
+    for j, at in zip(jobs, arrivals):
+        delay = at - (srv.now() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(**j))
+    for rid in ids:
+        srv.wait(rid)
+    infos = [srv.result(rid)[1] for rid in ids]
+    st = srv.stats()
+    srv.close()
+    lat = sorted(i["finish_s"] - i["submit_s"] for i in infos)
+    makespan = max(i["finish_s"] for i in infos) - min(i["submit_s"] for i in infos)
+    an_pairs = sum(i["units"] for j, i in zip(jobs, infos) if j["kind"] != cra.JOB_HAMMING_MC)
+    samples = sum(i["units"] for j, i in zip(jobs, infos) if j["kind"] == cra.JOB_HAMMING_MC)
+    pct = lambda q: lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]
+    return {
+        "workload": f"{njobs} requests, Poisson rate {rate}/s, mix of exhaustive AN k 18..24, grid AN k=32 "
+                    f"stride 4097, MC (72,64) 2^30 samples; seed {seed}",
+        "latency_p50_s": round(pct(0.5), 3), "latency_p99_s": round(pct(0.99), 3),
+        "makespan_s": round(makespan, 3),
+        "an_gpairs_per_s": round(an_pairs / makespan / 1e9, 2),
+        "mc_gsamples_per_s": round(samples / makespan / 1e9, 3),
+        "iterations": st["iterations"], "device_busy_frac": round(st["busy_device_ms"] / 1e3 / makespan, 3),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +202,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--rows-per-lane", type=int, default=0)
     ap.add_argument("--chunk-blocks", type=int, default=0)
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-per-launch comparison")
+    ap.add_argument("--served", action="store_true", help="also run the served request mix")
+    ap.add_argument("--served-jobs", type=int, default=24)
+    ap.add_argument("--served-rate", type=float, default=1.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -142,6 +265,16 @@ def main() -> int:
         "peak_clock_ghz": PEAK_CLOCK_GHZ,
     }
 
+    batched = None
+    if not args.no_batched:
+        log("batched vs per-launch ...")
+        batched = run_batched(ctx)
+
+    served = None
+    if args.served:
+        log("served mix ...")
+        served = run_served(ctx, args.served_jobs, args.served_rate)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -166,6 +299,8 @@ def main() -> int:
                    "A": A, "k": k, "pairs_per_step": pairs_per_step, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
     ctx.close()
