@@ -60,6 +60,23 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+CLIS = {"ANcandidates": "ancandidates_main.cpp", "codrel_supera": "supera_main.cpp"}
+
+
+def build_clis() -> list[str]:
+    """The ANcandidates and super-A command lines over the C ABI, into _build/."""
+    os.makedirs(BUILD, exist_ok=True)
+    out = []
+    for name, src in CLIS.items():
+        exe = os.path.join(BUILD, name)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"),
+               os.path.join(PKG, "cli", src), "-L", PKG, "-lcodrel",
+               "-Wl,-rpath,$ORIGIN/../coding_reliability_amd", "-o", exe]
+        subprocess.run(cmd, check=True)
+        out.append(exe)
+    return out
+
+
 REFERENCE = os.environ.get("CR_REFERENCE_ROOT", "/root/reference")
 REF_CLI = os.path.join(BUILD, "codrel_ref")
 
@@ -92,6 +109,7 @@ def dump_isa(src_name: str, out_dir: str) -> str:
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_clis()
     build_reference_cli()
     if len(sys.argv) > 2 and sys.argv[1] == "--isa":
         print(dump_isa(sys.argv[2], os.path.join(ROOT, "_work", "isa")))

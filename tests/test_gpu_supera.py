@@ -81,3 +81,37 @@ def test_an_candidates_tsv_layout(ctx):
     assert lines[0] == "# bit width data = [6,8]" and len(lines) == 3 + 3 * 2
     n, k, a, best, cnt, lst = lines[3].split("\t")
     assert int(n) == int(k) + int(a) and int(cnt) == len(lst.split(","))
+
+
+import os
+import subprocess
+
+from conftest import ROOT
+
+BUILD = os.path.join(ROOT, "_build")
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BUILD, "ANcandidates")), reason="CLI not built")
+def test_ancandidates_cli(ctx):
+    out = subprocess.run([os.path.join(BUILD, "ANcandidates"), "-d", "6", "-D", "9", "-a", "4", "-A", "6"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = out.stdout.strip().split("\n")
+    assert lines[2] == "# n=|C|\tk=|D|\t|A|\t|SDR|\t|candidates|\tcandidates"
+    for ln in lines[3:]:
+        n, k, a, best, cnt, lst = ln.split("\t")
+        ob, oc = O.an_candidates(int(k), int(a))
+        assert int(best) == ob and [int(x) for x in lst.split(",")] == oc and int(cnt) == len(oc)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(BUILD, "codrel_supera")), reason="CLI not built")
+def test_supera_cli(ctx, tmp_path):
+    r = subprocess.run([os.path.join(BUILD, "codrel_supera"), "-k", "10", "-s", "33", "-S", "63", "-f", "t"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    body = (tmp_path / "t_k10_a33-63.csv").read_text().strip().split("\n")
+    ref, _ = ctx.supera_search(1, 10, A_start=33, A_end=63)
+    # identical apart from the time fields
+    strip_t = lambda rows: [",".join(x.split(",")[:-11]) for x in rows[3:]]
+    assert strip_t(body) == strip_t(ref.strip().split("\n"))
+    assert len(body) == 3 + 16
