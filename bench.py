@@ -191,6 +191,23 @@ def run_served(ctx, njobs: int, rate: float, seed: int = 7) -> dict:
     }
 
 
+def traffic_probe() -> int:
+    """For rocprofv3 --pmc runs: one device copy of exactly 1 GiB (torch, to
+    scale FETCH_SIZE / WRITE_SIZE into bytes) followed by one pair-kernel
+    launch (A=58659, k=20)."""
+    import torch
+    import coding_reliability_amd as cra
+    x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    ctx = cra.Context(0)
+    raw, ms = ctx.an_pairs(58659, 1 << 20)
+    log(f"copy 1 GiB done; pair kernel A=58659 k=20: {ms:.3f} ms, sum {sum(raw)}")
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,7 +223,10 @@ def main() -> int:
     ap.add_argument("--served", action="store_true", help="also run the served request mix")
     ap.add_argument("--served-jobs", type=int, default=24)
     ap.add_argument("--served-rate", type=float, default=1.0)
+    ap.add_argument("--traffic-probe", action="store_true", help="copy + one pair launch, for PMC counter runs")
     args = ap.parse_args()
+    if args.traffic_probe:
+        return traffic_probe()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "

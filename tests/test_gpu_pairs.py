@@ -137,3 +137,26 @@ def test_invalid_arguments(ctx):
         ctx.an_pairs(61, 1 << 16, nbins=5)
     with pytest.raises(cra.CRError):
         ctx.an_pairs((1 << 63) + 1, 4)
+
+
+def test_all_exact_fixtures_k_le_20_in_one_batch(ctx):
+    """Every exact exhaustive result file of the reference with k <= 20 (A odd
+    1..127 at k=8..16, A=35..63 at k=18, four at k=20, A=61 runtimes) in one
+    batched launch, bit-exact in the reported form."""
+    runs = [r for r in golden("an_exhaustive_runs.json")["runs"] if r["n"] <= 20]
+    assert len(runs) >= 600
+    codes = [((r["A"] if r["A"] & (r["A"] - 1) else 1), 1 << r["n"]) for r in runs]
+    got, ms = ctx.an_batch(codes)
+    print(f"{len(codes)} recorded codes in one launch: {ms:.1f} ms", flush=True)
+    for r, raw in zip(runs, got):
+        assert report(r["n"], raw + [0] * 64, len(r["counts"])) == r["counts"], r["src"]
+
+
+def test_more_k24_fixtures_batched(ctx):
+    # results/As/ancoding_gpu_A17_n24_h5.csv (30-bit words) and A55831 (40-bit)
+    runs = [r for r in golden("an_exhaustive_runs.json")["runs"] if r["n"] == 24 and r["A"] in (17, 55831)]
+    assert len(runs) == 2
+    got, ms = ctx.an_batch([(r["A"], 1 << 24) for r in runs])
+    print(f"two k=24 codes: {ms:.1f} ms", flush=True)
+    for r, raw in zip(runs, got):
+        assert report(24, raw + [0] * 64, len(r["counts"])) == r["counts"], r["src"]

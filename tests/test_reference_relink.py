@@ -63,3 +63,31 @@ def test_cli_hamming_matches_golden(tmp_path, solutions):
     got = read_counts(tmp_path / "hamming_gpu_n24_h6.csv")
     sol = solutions["sol_hamming_n24"]
     assert all(got[i] == sol[i] << 24 for i in range(4, len(sol), 2))
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+def test_cli_hamming_grid_matches_oracle(tmp_path):
+    # run_hamming_grid (hamming_grid.cu:91-219): k=32, m=2^20 sample points,
+    # even bins = 2^32 * (w[i] + w[i-1]) over the reference's sample set
+    from oracle import oracle as O
+    k, m = 32, 1 << 20
+    run_cli(tmp_path, "-e", "1", "-n", str(k), "-m", str(m), "-g", "1", "-f")
+    got = read_counts(tmp_path / f"hamming_mc_m{m}_n{k}_h7.csv")
+    words = O.hamming_grid_ref_samples(m, float(1 << k) / m, 256)
+    w = O.hamming_weights_list(k, words)
+    for i in range(2, 40, 2):
+        assert got[i] == (w[i] + w[i - 1]) << k, i
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+def test_cli_an_mc_close_to_golden(tmp_path, solutions):
+    # run_ancoding_mc with Philox samples: scaled counts sum to ~4^k and stay
+    # close to the golden A=61, k=16 vector
+    out = run_cli(tmp_path, "-a", "61", "-n", "16", "-m", "4096", "-f")
+    got = read_counts(tmp_path / "ancoding_mc_gpu_m4096_A61_n16_h6.csv")
+    sol = solutions["solution_an16_A61"]
+    assert abs(sum(got) - 4 ** 16) < 4 ** 16 * 1e-6 + 64
+    for i in range(5, 20):
+        assert abs(got[i] - sol[i]) <= 0.02 * sol[i], i
