@@ -45,7 +45,7 @@ u64p = C.POINTER(C.c_uint64)
 
 def build() -> str:
     """Compile liboracle.so in place (gcc, no GPU)."""
-    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    subprocess.run(["make", "-s", "-B", "-C", _HERE], check=True)
     return os.path.join(_HERE, "liboracle.so")
 
 
