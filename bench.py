@@ -302,7 +302,7 @@ def main() -> int:
         log(f"cpu baseline {cpu['value']} Gpairs/s on {cpu['cores']} threads")
 
     out = {
-        "metric": "pair-comparisons/sec (Gpairs/s), exhaustive AN histogram",
+        "metric": "pair-comparisons/sec (Gpairs/s) + %VALU peak at k=24; batched vs per-launch speedup",
         "value": round(gpairs, 3),
         "unit": "Gpairs/s",
         "n_gpus": 1,
@@ -312,7 +312,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32/u64 integer (exact)",
+        "dtype": "u32",
         "data": "synthetic (codewords c=A*d generated on the GPU)",
         "config": {"workload": f"exhaustive AN histogram, A={A}, k={k}, {width_bits}-bit codewords, "
                                f"all C(2^{k},2) pairs, {width_bits + 1} bins",

@@ -47,7 +47,10 @@ int main(int argc, char **argv) {
     }
     std::vector<uint64_t> As;
     if (mode == 2) {
-        // comma-separated values per line (super_a/src/main.cpp:172-192)
+        // Comma-separated values per line (super_a/src/main.cpp:172-192).  The
+        // ANcandidates output is accepted as well: '#' lines are comments and a
+        // tab-separated row contributes its last column, the candidate list
+        // (ANcandidates.cpp:286-299).
         std::ifstream fp(file);
         if (!fp.good()) {
             std::cerr << "Candidates file does not exist.\n";
@@ -55,9 +58,15 @@ int main(int argc, char **argv) {
         }
         std::string line, tok;
         while (std::getline(fp, line)) {
+            if (!line.empty() && line[0] == '#') continue;
+            const size_t tab = line.rfind('\t');
+            if (tab != std::string::npos) line = line.substr(tab + 1);
             std::stringstream ss(line);
-            while (std::getline(ss, tok, ','))
-                if (!tok.empty()) As.push_back(std::stoull(tok));
+            while (std::getline(ss, tok, ',')) {
+                const size_t b = tok.find_first_not_of(" \r");
+                if (b == std::string::npos) continue;
+                As.push_back(std::stoull(tok.substr(b)));
+            }
         }
     }
     cr_context *ctx = nullptr;

@@ -34,6 +34,14 @@ def test_all_grid1d_fixtures(ctx):
         if "/runtimes/" in r["src"] and m == 1000:
             # earlier driver version: counts[0] = 2^k was not overwritten
             assert rep[1:] == r["counts"][1:], r["src"]
+        elif "/runtimes/" in r["src"] and m == 10000 and n == 32:
+            # earlier driver version: factor and product in double, not long
+            # double (these 14 files differ from the long double product by up
+            # to 180 in bins 8..30, and equal the double product in every bin)
+            f = 2.0 ** n / m
+            want = [1 << n] + [int(f * float(hist[i] if i < len(hist) else 0)) for i in range(1, cc)]
+            assert want == r["counts"], r["src"]
+            assert rep[0] == r["counts"][0]
         else:
             assert rep == r["counts"], r["src"]
 

@@ -115,3 +115,21 @@ def test_supera_cli(ctx, tmp_path):
     strip_t = lambda rows: [",".join(x.split(",")[:-11]) for x in rows[3:]]
     assert strip_t(body) == strip_t(ref.strip().split("\n"))
     assert len(body) == 3 + 16
+
+
+@pytest.mark.skipif(not (os.path.exists(os.path.join(BUILD, "codrel_supera"))
+                         and os.path.exists(os.path.join(BUILD, "ANcandidates"))), reason="CLI not built")
+def test_supera_cli_reads_ancandidates_tsv(ctx, tmp_path):
+    tsv = subprocess.run([os.path.join(BUILD, "ANcandidates"), "-d", "8", "-D", "8", "-a", "6", "-A", "7"],
+                         capture_output=True, text=True, timeout=120)
+    assert tsv.returncode == 0, tsv.stderr
+    (tmp_path / "cands.tsv").write_text(tsv.stdout)
+    want = []
+    for ln in tsv.stdout.strip().split("\n"):
+        if not ln.startswith("#"):
+            want += [int(x) for x in ln.split("\t")[-1].split(",")]
+    r = subprocess.run([os.path.join(BUILD, "codrel_supera"), "-k", "8", "-c", "cands.tsv", "-f", "t"],
+                       cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    body = (tmp_path / "t_k8_c_cands.tsv.csv").read_text().strip().split("\n")
+    assert [int(x.split(",")[2]) for x in body[3:]] == want
