@@ -54,13 +54,29 @@ def cpu_baseline(A: int, k: int, target_s: float) -> dict:
     from oracle import oracle as O
     O.lib()
     threads = O.num_threads()
-    # calibrate on one shard (64 rows x 2^k columns)
-    t0 = time.perf_counter()
-    _, pairs = O.an_cpu_counts(k, A, 0, 1)
-    dt = max(time.perf_counter() - t0, 1e-6)
-    shards = max(1, int(target_s / dt))
-    nshards_total = (1 << k) // 64
-    shards = min(shards, nshards_total)
+    nshards_total = max(1, (1 << k) // 64)
+    # calibrate: grow the leading row-shard range until it takes >= 1/8 of the
+    # target, then scale it to the target (shards near the top hold the most
+    # pairs, so the per-pair rate is what scales, not the shard count)
+    shards = min(nshards_total, 4 * threads)
+    while True:
+        t0 = time.perf_counter()
+        _, pairs = O.an_cpu_counts(k, A, 0, shards)
+        dt = max(time.perf_counter() - t0, 1e-6)
+        if dt >= target_s / 8 or shards >= nshards_total:
+            break
+        shards = min(nshards_total, shards * 4)
+    want_pairs = pairs / dt * target_s
+    lo, hi = shards, nshards_total
+    N = 1 << k
+    while lo < hi:  # smallest shard count whose pair total reaches want_pairs
+        mid = (lo + hi) // 2
+        rows = min(N, 64 * mid)
+        if rows * N - rows * (rows + 1) // 2 >= want_pairs:
+            hi = mid
+        else:
+            lo = mid + 1
+    shards = lo
     t0 = time.perf_counter()
     _, pairs = O.an_cpu_counts(k, A, 0, shards)
     dt = time.perf_counter() - t0
@@ -191,21 +207,42 @@ def run_served(ctx, njobs: int, rate: float, seed: int = 7) -> dict:
     }
 
 
-def traffic_probe() -> int:
+def traffic_probe(A: int, k: int) -> int:
     """For rocprofv3 --pmc runs: one device copy of exactly 1 GiB (torch, to
     scale FETCH_SIZE / WRITE_SIZE into bytes) followed by one pair-kernel
-    launch (A=58659, k=20)."""
+    launch of the headline problem (A, k)."""
     import torch
     import coding_reliability_amd as cra
     x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
     y = torch.empty_like(x)
     y.copy_(x)
     torch.cuda.synchronize()
+    log("copy of 1 GiB done")
     ctx = cra.Context(0)
-    raw, ms = ctx.an_pairs(58659, 1 << 20)
-    log(f"copy 1 GiB done; pair kernel A=58659 k=20: {ms:.3f} ms, sum {sum(raw)}")
+    raw, ms = ctx.an_pairs(A, 1 << k)
+    log(f"pair kernel A={A} k={k}: {ms:.3f} ms, sum {sum(raw)}")
     ctx.close()
     return 0
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pair_kernel_A58659_k24_hbm_traffic.txt")
+
+
+def measured_traffic(A: int, k: int):
+    """HBM bytes per pair-kernel launch from the committed FETCH_SIZE /
+    WRITE_SIZE summary (counter values scaled by the same counters read on a
+    1 GiB copy), or None when no summary exists for this (A, k)."""
+    try:
+        kv = {}
+        for line in open(TRAFFIC_FILE):
+            if ":" in line and not line.startswith("#"):
+                key, val = line.split(":", 1)
+                kv[key.strip()] = val.strip()
+        if int(kv["A"]) != A or int(kv["k"]) != k:
+            return None
+        return int(float(kv["traffic_bytes"]))
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main() -> int:
@@ -226,7 +263,7 @@ def main() -> int:
     ap.add_argument("--traffic-probe", action="store_true", help="copy + one pair launch, for PMC counter runs")
     args = ap.parse_args()
     if args.traffic_probe:
-        return traffic_probe()
+        return traffic_probe(args.A, args.k)
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -277,7 +314,7 @@ def main() -> int:
         "peak": round(peak, 3),
         "unit": "Tops/s (int32 lane-ops)",
         "frac": round(achieved / peak, 4),
-        "traffic": None,
+        "traffic": measured_traffic(A, k),
         "kernel": "pair_kernel<R>",
         "kernel_ms": round(avg_kernel_s * 1e3, 3),
         "ops_per_pair": ops_per_pair,
