@@ -159,7 +159,7 @@ def run_batched(ctx, autotune: bool = True) -> dict:
     return out
 
 
-def run_served(ctx, njobs: int, rate: float, seed: int = 7) -> dict:
+def run_served(ctx, njobs: int, rate: float, seed: int = 7, max_wait_s: float = 300.0) -> dict:
     """C5: Poisson arrivals (rate jobs/s) of a uniform mix of exhaustive AN
     (k in 18..24, odd 16-bit A), integer-grid AN (k=32, stride 4097, odd 16-bit
     A) and Monte Carlo (72,64) Extended Hamming (2^30 samples) requests."""
@@ -186,8 +186,22 @@ def run_served(ctx, njobs: int, rate: float, seed: int = 7) -> dict:
         if delay > 0:
             time.sleep(delay)
         ids.append(srv.submit(**j))
+    deadline = time.perf_counter() + max_wait_s
     for rid in ids:
-        srv.wait(rid)
+        while True:
+            try:
+                srv.wait(rid, timeout_ms=15000)
+                break
+            except cra.CRError as e:
+                if e.status != 9:  # CR_ERR_BUSY: still running
+                    raise
+                st = srv.stats()
+                log(f"served: waiting for request {rid}: {st['completed']}/{len(ids)} done, "
+                    f"{st['active']} active, {st['queued']} queued, {st['iterations']} iterations")
+                if time.perf_counter() > deadline:
+                    srv.close()
+                    raise RuntimeError(f"served mix unfinished after {max_wait_s} s")
+        log(f"served: request {rid} done")
     infos = [srv.result(rid)[1] for rid in ids]
     st = srv.stats()
     srv.close()

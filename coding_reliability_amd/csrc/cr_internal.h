@@ -149,6 +149,8 @@ static_assert(sizeof(HamJob) == 144, "HamJob layout");
 // returns the number of non-overall parity bits.
 uint32_t hamming_masks(uint32_t k, uint64_t masks[7]);
 uint64_t hamming_items(uint64_t count);
+// Waves per CU of the Hamming kernel (the larger of the 32- and 64-bit forms).
+uint32_t hamming_waves_per_cu(uint32_t lds_rows);
 cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_items, bool w64,
                          unsigned long long *counter, uint32_t lds_rows, int cus, hipStream_t stream, hipEvent_t ev0,
                          hipEvent_t ev1);

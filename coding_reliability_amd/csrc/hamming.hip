@@ -11,6 +11,8 @@
 //   mc      d = Philox word s of (seed, stream) (replaces dhamming_mc's cuRAND draw)
 // AN Monte Carlo: every word A*i against m Philox samples A*u_ij
 // (replaces dancoding_mc_shared, an_coding_mc.cu:94-158).
+#include <algorithm>
+
 #include "cr_device.h"
 #include "cr_internal.h"
 
@@ -201,6 +203,18 @@ cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_it
 }
 
 uint64_t hamming_items(uint64_t count) { return (count + kItemSamples - 1) / kItemSamples; }
+
+uint32_t hamming_waves_per_cu(uint32_t lds_rows) {
+    const size_t lds = (size_t)lds_rows * kStride * sizeof(uint32_t);
+    int a = 0, b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, reinterpret_cast<const void *>(&ham_kernel<false>), 64,
+                                                     lds) != hipSuccess)
+        a = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void *>(&ham_kernel<true>), 64,
+                                                     lds) != hipSuccess)
+        b = 0;
+    return (uint32_t)std::max(std::max(a, b), 1);
+}
 
 cr_status launch_an_mc(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, unsigned long long *counter,
                        unsigned long long *out, uint32_t nbins, int cus, hipStream_t stream, hipEvent_t ev0,

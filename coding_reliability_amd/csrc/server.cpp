@@ -6,7 +6,11 @@
 //      a histogram slot in the server's device slab, zeroed on the device;
 //   2. gives every active request a slice of its remaining work items, sized
 //      from a measured throughput model so the iteration lasts about
-//      iteration_ms of device time, shared equally among active requests;
+//      iteration_ms of device time, shared equally among active requests,
+//      but never fewer than two items per wave of the launch (a launch with
+//      fewer items than waves runs as long as one item however few there
+//      are, so a smaller slice would only lower the measured rate and shrink
+//      the next slice further);
 //   3. launches one pair-kernel pass over the AN slices and one Hamming pass
 //      over the Monte Carlo slices (each kernel works on all slices at once);
 //   4. retires finished requests: copies their histogram to the host and
@@ -57,7 +61,7 @@ struct cr_server {
     cr_context *ctx = nullptr;
     cr_server_config cfg{};
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
     DeviceBuffer slab, pair_jobs, ham_jobs, counter;
     HostBuffer staging, readback;
     std::vector<int> free_slots;
@@ -72,8 +76,8 @@ struct cr_server {
     bool stop = false;
     std::thread worker;
     std::chrono::steady_clock::time_point t0;
-    long double pair_rate = 5e9L;    // pairs per device ms (refined online)
-    long double sample_rate = 2e9L;  // Hamming samples per device ms
+    long double pair_rate = 8e9L;    // pairs per device ms (refined online)
+    long double sample_rate = 1e9L;  // Hamming samples per device ms
 
     double now() const {
         return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -106,9 +110,9 @@ cr_status describe(cr_server *s, const cr_job &j, Request &r) {
         p.N = N;
         p.mode = pair_mode(A, N);
         p.nbins = pair_max_distance(A, N) + 1;
-        // items of about 2^27 pairs
+        // items of about 2^24 pairs (a few ms for one wave)
         const uint64_t RB = 64ull * R;
-        uint64_t Q = std::max<uint64_t>((1ull << 27) / (RB * RB), 1);
+        uint64_t Q = std::max<uint64_t>((1ull << 24) / (RB * RB), 1);
         pair_job_geometry(p, R, (uint32_t)Q);
         r.nbins = p.nbins;
         r.total_items = p.items;
@@ -147,12 +151,26 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
     std::vector<std::pair<Request *, uint64_t>> took;
     uint64_t ptotal = 0, htotal = 0;
     uint32_t lds_pair = 2, lds_ham = 2;
-    bool ham64 = false;
+    uint64_t n_an = 0, n_mc = 0;
+    for (Request *r : run) {
+        if (r->kind == CR_JOB_HAMMING_MC) {
+            lds_ham = std::max(lds_ham, r->hj.nbins + 1);
+            ++n_mc;
+        } else {
+            lds_pair = std::max(lds_pair, r->pj.nbins + 1);
+            ++n_an;
+        }
+    }
+    // waves each launch runs with: a slice floor of two items per wave
+    const uint64_t wp = n_an ? (uint64_t)pair_waves_per_cu(cfg.rows_per_lane, lds_pair) * ctx->cus : 0;
+    const uint64_t wh = n_mc ? (uint64_t)hamming_waves_per_cu(lds_ham) * ctx->cus : 0;
     long double pair_units = 0, ham_units = 0;
     for (Request *r : run) {
-        const long double rate = r->kind == CR_JOB_HAMMING_MC ? sample_rate : pair_rate;
+        const bool mc = r->kind == CR_JOB_HAMMING_MC;
+        const long double rate = mc ? sample_rate : pair_rate;
         uint64_t n = (uint64_t)(share * rate / std::max<long double>(r->units_per_item, 1));
-        n = std::max<uint64_t>(n, 1);
+        const uint64_t floor_n = mc ? (2 * wh + n_mc - 1) / n_mc : (2 * wp + n_an - 1) / n_an;
+        n = std::max<uint64_t>({n, floor_n, 1});
         n = std::min<uint64_t>(n, r->total_items - r->cursor);
         unsigned long long *out = slab.as<unsigned long long>() + (size_t)r->slot * kSlotBins;
         if (r->kind == CR_JOB_HAMMING_MC) {
@@ -162,8 +180,6 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
             h.items = n;
             h.item_offset = r->cursor;
             htotal += n;
-            lds_ham = std::max(lds_ham, h.nbins + 1);
-            ham64 = ham64 || h.k > 32;
             hjobs.push_back(h);
             ham_units += n * r->units_per_item;
         } else {
@@ -173,7 +189,6 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
             p.items = n;
             p.item_offset = r->cursor;
             ptotal += n;
-            lds_pair = std::max(lds_pair, p.nbins + 1);
             pjobs.push_back(p);
             pair_units += n * r->units_per_item;
         }
@@ -213,6 +228,7 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
         CR_TRY(launch_pairs(pair_jobs.as<PairJob>(), (uint32_t)pjobs.size(), ptotal, ctr, pc, ctx->cus, stream,
                             nullptr, nullptr));
     }
+    CR_HIP(hipEventRecord(evm, stream));
     if (!h32.empty())
         CR_TRY(launch_hamming(ham_jobs.as<HamJob>(), (uint32_t)h32.size(), t32, false, ctr + 1, lds_ham, ctx->cus,
                               stream, nullptr, nullptr));
@@ -221,18 +237,16 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
                               ctx->cus, stream, nullptr, nullptr));
     CR_HIP(hipEventRecord(ev1, stream));
     CR_HIP(hipStreamSynchronize(stream));
-    float ms = 0;
+    float ms = 0, pms = 0, hms = 0;
     CR_HIP(hipEventElapsedTime(&ms, ev0, ev1));
-    (void)ham64;
-    // refine the throughput model from this iteration (exponential average)
-    if (ms > 0.05f) {
-        const long double tot = pair_units / pair_rate + ham_units / sample_rate;
-        if (tot > 0) {
-            const long double scale = tot / ms;
-            pair_rate = 0.7L * pair_rate + 0.3L * pair_rate * scale;
-            sample_rate = 0.7L * sample_rate + 0.3L * sample_rate * scale;
-        }
-    }
+    CR_HIP(hipEventElapsedTime(&pms, ev0, evm));
+    CR_HIP(hipEventElapsedTime(&hms, evm, ev1));
+    // refine the throughput model (exponential average) from launches that
+    // had at least one item per wave, i.e. ran at the device's full width
+    if (!pjobs.empty() && ptotal >= wp && pms > 0.05f)
+        pair_rate = 0.7L * pair_rate + 0.3L * (pair_units / pms);
+    if (!hjobs.empty() && htotal >= wh && hms > 0.05f)
+        sample_rate = 0.7L * sample_rate + 0.3L * (ham_units / hms);
     std::lock_guard<std::mutex> lk(mu);
     st.iterations++;
     st.busy_device_ms += ms;
@@ -350,6 +364,7 @@ cr_status cr_server_create(cr_context *ctx, const cr_server_config *cfg, cr_serv
         return fail(CR_ERR_INVALID_ARGUMENT, "rows_per_lane must be 2, 4, 8 or 16");
     CR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     CR_HIP(hipEventCreate(&s->ev0));
+    CR_HIP(hipEventCreate(&s->evm));
     CR_HIP(hipEventCreate(&s->ev1));
     CR_TRY(s->slab.reserve((size_t)s->cfg.max_active * kSlotBins * sizeof(uint64_t)));
     CR_TRY(s->counter.reserve(4 * sizeof(uint64_t)));
@@ -471,6 +486,7 @@ cr_status cr_server_destroy(cr_server *s) {
     s->staging.release();
     s->readback.release();
     if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->evm) (void)hipEventDestroy(s->evm);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;

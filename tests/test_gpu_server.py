@@ -114,3 +114,20 @@ def test_invalid_requests(ctx):
             srv.submit(cra.JOB_AN_EXHAUSTIVE, 0, A=3)
         with pytest.raises(cra.CRError):
             srv.result(12345)
+
+
+def test_large_request_runs_at_device_rate(ctx):
+    """A request of 8.8e12 pairs (k=22) served alone finishes in a small
+    multiple of the one-shot kernel time: each iteration fills the device (at
+    least two work items per wave), so the slice size cannot shrink towards a
+    handful of items per iteration."""
+    j = dict(kind=cra.JOB_AN_EXHAUSTIVE, k=22, A=58659)
+    ref, kms = ctx.an_pairs(j["A"], 1 << j["k"])
+    with cra.Server(ctx, iteration_ms=20) as srv:
+        t0 = time.perf_counter()
+        rid = srv.submit(**j)
+        srv.wait(rid, timeout_ms=60000)
+        dt = time.perf_counter() - t0
+        got, info = srv.result(rid)
+    assert got == ref[: len(got)]
+    assert dt < 3 * kms / 1e3 + 1.0, (dt, kms, info)
