@@ -18,7 +18,7 @@ JOB_AN_EXHAUSTIVE, JOB_AN_GRID, JOB_HAMMING_MC = 0, 1, 2
 HAM_RANGE, HAM_STRIDE, HAM_LIST, HAM_MC = 0, 1, 2, 3
 
 __all__ = ["Context", "CRError", "an_bins", "ceil_log2", "grid_report", "hamming_parity_bits", "hamming_N",
-           "hamming_report", "HAM_RANGE", "HAM_STRIDE", "HAM_LIST", "HAM_MC"]
+           "hamming_report", "hamming_sdc_table", "HAM_RANGE", "HAM_STRIDE", "HAM_LIST", "HAM_MC"]
 
 
 def hamming_parity_bits(k: int) -> int:
@@ -39,6 +39,24 @@ def hamming_report(k: int, weights: list[int], with_1bit: bool) -> list[int]:
     out = np.zeros(2 * (N + 1), dtype=np.uint64)
     check(lib().cr_hamming_report(k, _u64p(w), int(with_1bit), _u64p(out)))
     return _u128(out)
+
+
+def hamming_sdc_table(k: int, method: int = 2, ctx: "Context | None" = None) -> tuple[str, list[int]]:
+    """SDC table text of the (N,k) Extended Hamming code in the legacy table
+    layout (legacy/hamming_count_undetectable_errors.cpp:226-339) and the
+    non-extended weight distribution it was built from.  method 1 enumerates
+    all 2^k words on the device (k <= 40, needs ctx), 2 uses the MacWilliams
+    identity, 0 picks."""
+    N = hamming_N(k)
+    w = np.zeros(2 * (N + 1), dtype=np.uint64)
+    txt = C.c_void_p()
+    check(lib().cr_hamming_sdc_table(ctx.handle if ctx is not None else None, k, method, _u64p(w), N + 1,
+                                      C.byref(txt)))
+    try:
+        s = C.string_at(txt).decode()
+    finally:
+        lib().cr_free(txt)
+    return s, _u128(w)
 
 
 def _u64p(a):
@@ -165,6 +183,10 @@ class Context:
         finally:
             lib().cr_free(txt)
         return s, secs.value
+
+    def hamming_sdc_table(self, k: int, method: int = 0) -> tuple[str, list[int]]:
+        """hamming_sdc_table on this context's device (method 0/1)."""
+        return hamming_sdc_table(k, method, self)
 
     def an_candidates(self, k: int, abits: int) -> tuple[int, list[int], float]:
         """(best minimum NAF weight, sorted A attaining it, kernel ms) for one

@@ -96,6 +96,7 @@ SIGNATURES = {
     "cr_hamming_weights": (C.c_int, [C.c_void_p, C.POINTER(cr_ham_request), u64p, C.c_uint32,
                                      C.POINTER(C.c_double)]),
     "cr_hamming_report": (C.c_int, [C.c_uint32, u64p, C.c_int, u64p]),
+    "cr_hamming_sdc_table": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, u64p, C.c_uint32, C.POINTER(C.c_void_p)]),
     "cr_an_batch": (C.c_int, [C.c_void_p, C.POINTER(cr_an_code), C.c_uint32, u64p, C.c_uint32,
                               C.POINTER(cr_pair_tuning), C.POINTER(C.c_double)]),
     "cr_batch_plan_create": (C.c_int, [C.c_void_p, C.POINTER(cr_an_code), C.c_uint32, C.c_uint32,

@@ -60,11 +60,13 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-CLIS = {"ANcandidates": "ancandidates_main.cpp", "codrel_supera": "supera_main.cpp"}
+CLIS = {"ANcandidates": "ancandidates_main.cpp", "codrel_supera": "supera_main.cpp",
+        "hamming_tables": "hamming_tables_main.cpp"}
 
 
 def build_clis() -> list[str]:
-    """The ANcandidates and super-A command lines over the C ABI, into _build/."""
+    """The ANcandidates, super-A and Hamming-table command lines over the C ABI,
+    into _build/."""
     os.makedirs(BUILD, exist_ok=True)
     out = []
     for name, src in CLIS.items():

@@ -110,7 +110,11 @@ extern "C" cr_status cr_hamming_report(uint32_t k, const uint64_t *weights, int 
         for (uint32_t i = 3; i < cc; i += 2)
             c[i] = i + 1 < cc ? (unsigned __int128)(i + 1) * c[i + 1] + (unsigned __int128)(N - i + 1) * c[i - 1]
                               : (unsigned __int128)(N - i + 1) * c[i - 1];
-    for (uint32_t i = 3; i < cc; ++i) c[i] <<= k;
+    for (uint32_t i = 3; i < cc; ++i) {
+        if (c[i] >> (128 - k))
+            return fail(CR_ERR_UNSUPPORTED, "reported count of weight %u exceeds 128 bits at k=%u", i, k);
+        c[i] <<= k;
+    }
     for (uint32_t i = 0; i < cc; ++i) {
         out_u128[2 * i] = (uint64_t)c[i];
         out_u128[2 * i + 1] = (uint64_t)(c[i] >> 64);

@@ -148,8 +148,22 @@ cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *req, uint64_
  * weight histogram: out[0] = 2^k, out[1] = N*2^k with spheres, odd weights
  * merged into the next even bin, the 1-bit-sphere closed form for odd bins,
  * bins >= 3 shifted by k.  N = k + parity bits + 1; out has N+1 uint128 values
- * as (lo, hi) pairs. */
+ * as (lo, hi) pairs.  CR_ERR_UNSUPPORTED when a count exceeds 128 bits (the
+ * reference's uint128_t; for such k cr_hamming_sdc_table keeps 256 bits). */
 cr_status cr_hamming_report(uint32_t k, const uint64_t *weights, int with_1bit, uint64_t *out_u128);
+
+/* SDC table of the (N,k) Extended Hamming code, k = 1..64, as printed by the
+ * legacy table program (legacy/hamming_count_undetectable_errors.cpp:226-339,
+ * its countHammingUndetectableErrors<k> output): per weight i = 0..N the
+ * non-extended weight distribution, the detecting-only and correcting Extended
+ * Hamming distributions, their totals x 2^k, 2^k * C(N, i) and the two
+ * probabilities.  method 0: device enumeration of all 2^k words for k <= 40,
+ * else the MacWilliams identity over the 2^r dual words (exact, any k); 1:
+ * device enumeration (k <= 40); 2: MacWilliams (ctx may be NULL).  If
+ * weights_u128 is not NULL it receives the N+1 non-extended weights as (lo, hi)
+ * pairs (nweights >= N+1).  *text is released with cr_free. */
+cr_status cr_hamming_sdc_table(cr_context *ctx, uint32_t k, int method, uint64_t *weights_u128, uint32_t nweights,
+                               char **text);
 
 /* --- AN Monte Carlo -------------------------------------------------------------
  * hist[b] = #{(i, j) : i < 2^n, j < m, popcount(A*i XOR A*u_ij) == b} with u_ij

@@ -13,6 +13,8 @@ Sources (paths relative to the reference root):
   distance_distribution/results/GridMs/*grid1d*.csv  deterministic grid1d runs
   distance_distribution/results/superAs/*.txt        super-A sweep summaries
   distance_distribution/legacy/print_codeword_matrix.out   k=5 full histograms
+  distance_distribution/legacy/hamming_count_undetectable_errors.cpp
+                                                     (40,33) and (52,45) weight tables
 
 A run file is kept only when its integer column is complete (the histogram
 sum matches the expected total); truncated runs are listed as excluded.
@@ -143,6 +145,26 @@ def codeword_matrices():
     return res
 
 
+def legacy_hamming_weights():
+    """The weight distributions of the non-extended (39,33) and (51,45) Hamming
+    codes that the legacy table program seeds its (40,33) and (52,45) runs with
+    (`current.counts[i] = ...` and the commented `current2.counts[i] = ...`)."""
+    p = os.path.join(DD, "legacy", "hamming_count_undetectable_errors.cpp")
+    lines = open(p).read().split("\n")
+    res = []
+    for var, k in (("current", 33), ("current2", 45)):
+        got, first, last = {}, None, None
+        for no, ln in enumerate(lines, 1):
+            m = re.match(r"\s*(?://\s*)?" + var + r"\.counts\[(\d+)\] = (\d+)(?:ull)?;", ln)
+            if m:
+                got[int(m.group(1))] = int(m.group(2))
+                first = first or no
+                last = no
+        counts = [got[i] for i in range(max(got) + 1)]
+        res.append(dict(src=f"{os.path.relpath(p, REF)}:{first}-{last}", k=k, hamming_bits=7, counts=counts))
+    return res
+
+
 def main():
     sol = parse_solutions()
     runs, excluded = exhaustive_runs()
@@ -152,6 +174,8 @@ def main():
     json.dump(dict(runs=grid_runs()), open(os.path.join(OUT, "an_grid1d_runs.json"), "w"))
     json.dump(dict(files=supera()), open(os.path.join(OUT, "supera_runs.json"), "w"))
     json.dump(dict(matrices=codeword_matrices()), open(os.path.join(OUT, "codeword_matrices.json"), "w"), indent=1)
+    json.dump(dict(tables=legacy_hamming_weights()), open(os.path.join(OUT, "legacy_hamming_weights.json"), "w"),
+              indent=1)
     print(f"solutions: {len(sol)} vectors; exhaustive runs kept {len(runs)}, excluded {len(excluded)}")
 
 
