@@ -159,6 +159,64 @@ def run_batched(ctx, autotune: bool = True) -> dict:
     return out
 
 
+# Algorithmic int32 lane-ops per unit of the other kernels (VALU-bound all):
+#   Hamming weight of a word: and + popcount per parity mask and data dword
+#   (2 r W), popcount of the data (W), one increment; Monte Carlo adds the
+#   Philox4x32-10 share (10 rounds x 8 ops per 4 words: 20 per 32-bit word,
+#   40 per 64-bit word).  Reference grid pair: xor + popcount per dword and an
+#   increment (2W + 1).  AN Monte Carlo pair: 2W + 1, the multiply, and the
+#   Philox share (20).  ANcandidates step (A, d): c += A, 3c, xor, popcount on
+#   64 bits (8) and a min (1).
+def _kernel_entry(units: float, ms: float, ops: float, unit: str, peak: float, workload: str) -> dict:
+    rate = units / (ms / 1e3)
+    achieved = rate * ops / 1e12
+    return {"workload": workload, "kernel_ms": round(ms, 3), "rate": round(rate / 1e9, 3), "rate_unit": unit,
+            "ops_per_unit": ops, "achieved_tops": round(achieved, 3), "frac": round(achieved / peak, 4)}
+
+
+def run_kernels(ctx, peak: float) -> dict:
+    """The other hot kernels on their workloads, each checked for a complete
+    histogram, timed with HIP events on the library's stream."""
+    import coding_reliability_amd as cra
+    out = {}
+    k = 32
+    h, ms = ctx.hamming_weights(k, cra.HAM_RANGE, 0, 1 << k)
+    assert sum(h) == 1 << k
+    out["hamming_exhaustive_39_32"] = _kernel_entry(1 << k, ms, 14, "Gwords/s", peak,
+                                                    "(39,32) weights of all 2^32 data words")
+    h, ms = ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 20, stride=65537)
+    assert sum(h) == 1 << 20
+    out["hamming_grid_39_32"] = _kernel_entry(1 << 20, ms, 14, "Gwords/s", peak,
+                                              "(39,32) integer grid d = i*65537 mod 2^32, i < 2^20")
+    S = 1 << 34
+    h, ms = ctx.hamming_weights(k, cra.HAM_MC, 0, S, seed=2026, stream=0)
+    assert sum(h) == S
+    out["hamming_mc_39_32"] = _kernel_entry(S, ms, 34, "Gwords/s", peak, "(39,32) 2^34 Philox words")
+    S = 1 << 30
+    h, ms = ctx.hamming_weights(64, cra.HAM_MC, 0, S, seed=2026, stream=1)
+    assert sum(h) == S
+    out["hamming_mc_72_64"] = _kernel_entry(S, ms, 71, "Gwords/s", peak, "(72,64) 2^30 Philox words")
+    h, ms = ctx.an_grid_ref(1, 32, 1001, 0, 61)
+    pairs = sum(h)
+    out["an_grid1d_ref"] = _kernel_entry(pairs, ms, 5, "Gpairs/s", peak,
+                                         f"reference grid 1D, A=61, k=32, m=1001 (38-bit codewords, {pairs} pairs)")
+    m = 1 << 16
+    h, ms = ctx.an_mc(61, 24, m, 16384)
+    assert sum(h) == (1 << 24) * m
+    out["an_mc"] = _kernel_entry((1 << 24) * m, ms, 24, "Gpairs/s", peak, "AN Monte Carlo, A=61, k=24, 2^16 samples")
+    best, cands, ms = ctx.an_candidates(24, 16)
+    # pruning skips most (A, d) steps, so the work is counted as the steps the
+    # candidates (A attaining the maximum, never pruned) must run in full: a
+    # lower bound of the executed steps, hence a lower bound of the fraction
+    steps = len(cands) * ((1 << 24) - 1)
+    e = _kernel_entry(steps, ms, 9, "Gsteps/s", peak,
+                      "ANcandidates cell k=24, |A|=16 (16384 odd A); rate = steps of the A attaining the maximum "
+                      "(each runs all 2^24-1 steps) per second, a lower bound")
+    e["best"], e["candidates"] = best, len(cands)
+    out["an_candidates"] = e
+    return out
+
+
 def run_served(ctx, njobs: int, rate: float, seed: int = 7, max_wait_s: float = 300.0) -> dict:
     """C5: Poisson arrivals (rate jobs/s) of a uniform mix of exhaustive AN
     (k in 18..24, odd 16-bit A), integer-grid AN (k=32, stride 4097, odd 16-bit
@@ -272,6 +330,8 @@ def main() -> int:
     ap.add_argument("--chunk-blocks", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-per-launch comparison")
     ap.add_argument("--served", action="store_true", help="also run the served request mix")
+    ap.add_argument("--no-kernels", action="store_true", help="skip the other kernels' measurements")
+    ap.add_argument("--kernels-only", action="store_true", help="only the other kernels (for counter runs)")
     ap.add_argument("--served-jobs", type=int, default=24)
     ap.add_argument("--served-rate", type=float, default=1.0)
     ap.add_argument("--traffic-probe", action="store_true", help="copy + one pair launch, for PMC counter runs")
@@ -293,6 +353,11 @@ def main() -> int:
     info = ctx.device_info()
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']} clock={info['max_clock_mhz']} MHz")
     tune = dict(rows_per_lane=args.rows_per_lane, chunk_blocks=args.chunk_blocks)
+    peak_tops = info["compute_units"] * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12
+    if args.kernels_only:
+        print(json.dumps({"kernels": run_kernels(ctx, peak_tops)}), flush=True)
+        ctx.close()
+        return 0
 
     for w in range(args.warmup):
         _, ms = ctx.an_pairs(A, N, **tune)
@@ -341,6 +406,13 @@ def main() -> int:
         log("batched vs per-launch ...")
         batched = run_batched(ctx)
 
+    kernels = None
+    if not args.no_kernels:
+        log("other kernels ...")
+        kernels = run_kernels(ctx, peak)
+        for name, e in kernels.items():
+            log(f"{name}: {e['rate']} {e['rate_unit']}, frac {e['frac']}")
+
     served = None
     if args.served:
         log("served mix ...")
@@ -371,6 +443,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "kernels": kernels,
         "served": served,
     }
     print(json.dumps(out), flush=True)

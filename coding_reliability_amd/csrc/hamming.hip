@@ -21,7 +21,10 @@ namespace {
 
 using namespace dev;
 
-constexpr uint32_t kSamplesPerLane = 64;  // per item
+// Samples per lane per work item: 32768 words per item keep the work-queue
+// counter (one device-scope atomic per item) far below its throughput limit
+// (about 8e7 items/s), which 4096-word items reached at 3.4e11 words/s.
+constexpr uint32_t kSamplesPerLane = 512;
 constexpr uint64_t kItemSamples = 64ull * kSamplesPerLane;
 
 template <bool W64>

@@ -68,7 +68,7 @@ def test_supera_candidate_list(ctx):
     assert [int(r[2]) for r in rows] == [58659, 63877, 59665]
 
 
-@pytest.mark.parametrize("k,abits", [(4, 3), (6, 4), (8, 5), (10, 6), (12, 7), (14, 9), (16, 8)])
+@pytest.mark.parametrize("k,abits", [(4, 3), (6, 4), (8, 5), (10, 6), (12, 7), (14, 9), (16, 8), (18, 12), (20, 10)])
 def test_an_candidates_vs_oracle(ctx, k, abits):
     best, cands, _ = ctx.an_candidates(k, abits)
     ob, oc = O.an_candidates(k, abits)
