@@ -27,7 +27,7 @@ namespace {
 using namespace dev;
 
 constexpr uint32_t kUnrollC = 8;
-constexpr uint64_t kCheck = 1u << 14;      // steps between early-exit checks
+constexpr uint64_t kCheck = 1u << 11;      // steps between early-exit checks
 constexpr uint64_t kMinChunk = 1u << 14;   // smallest d chunk worth an item
 
 __global__ void cand_init_kernel(uint32_t *__restrict__ vals, uint32_t *__restrict__ done, uint64_t n) {

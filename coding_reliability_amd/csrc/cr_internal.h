@@ -142,13 +142,17 @@ struct HamJob {
     unsigned long long *out;   // device weight histogram (nbins entries)
     uint64_t masks[7];         // parity masks truncated to k bits, zero when unused
     uint32_t source, k, stream, nbins;
+    uint32_t lane_samples;     // samples per lane per work item (multiple of 4, <= kHamMaxLaneSamples)
+    uint32_t pad;
 };
-static_assert(sizeof(HamJob) == 144, "HamJob layout");
+static_assert(sizeof(HamJob) == 152, "HamJob layout");
+constexpr uint32_t kHamMaxLaneSamples = 512;
 
 // Parity masks of the (N,k) extended Hamming codes, truncated to k bits;
 // returns the number of non-overall parity bits.
 uint32_t hamming_masks(uint32_t k, uint64_t masks[7]);
-uint64_t hamming_items(uint64_t count);
+// Work items of a job of `count` samples at lane_samples per lane (64 lanes).
+uint64_t hamming_items(uint64_t count, uint32_t lane_samples);
 // Waves per CU of the Hamming kernel (the larger of the 32- and 64-bit forms).
 uint32_t hamming_waves_per_cu(uint32_t lds_rows);
 cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_items, bool w64,

@@ -21,11 +21,10 @@ namespace {
 
 using namespace dev;
 
-// Samples per lane per work item: 32768 words per item keep the work-queue
-// counter (one device-scope atomic per item) far below its throughput limit
-// (about 8e7 items/s), which 4096-word items reached at 3.4e11 words/s.
-constexpr uint32_t kSamplesPerLane = 512;
-constexpr uint64_t kItemSamples = 64ull * kSamplesPerLane;
+// Work item = 64 lanes x job.lane_samples samples.  Up to 32768 words per item
+// keep the work-queue counter (one device-scope atomic per item) far below its
+// throughput limit (about 8e7 items/s, reached with 4096-word items at 3.4e11
+// words/s); small jobs use smaller items so that they still fill the device.
 
 template <bool W64>
 __device__ __forceinline__ uint32_t weight_nonext(uint64_t d, const uint64_t (&m)[7]) {
@@ -69,23 +68,24 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
         if (t >= total_items) break;
         const uint32_t jb = find_job(jobs, njobs, t, cur);
         const HamJob &job = jobs[jb];
-        if (jb != cur || pending + kSamplesPerLane >= 0xFFFFFFFFull) {
+        const uint32_t spl = job.lane_samples;
+        if (jb != cur || pending + spl >= 0xFFFFFFFFull) {
             if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, jobs[cur].out);
             cur = jb;
             pending = 0;
         }
-        pending += kSamplesPerLane;
+        pending += spl;
         uint64_t m[7];
 #pragma unroll
         for (int i = 0; i < 7; ++i) m[i] = job.masks[i];
         const uint64_t dmask = job.k >= 64 ? ~0ull : ((1ull << job.k) - 1);
-        const uint64_t base = (t - job.item_begin + job.item_offset) * kItemSamples;  // first sample of the item
+        const uint64_t base = (t - job.item_begin + job.item_offset) * (64ull * spl);  // first sample of the item
         const uint64_t count = job.count;
 
         if (job.source == kHamMc) {
             const uint32_t per = job.k <= 32 ? 4 : 2;  // words per Philox block
             const uint32_t k0 = (uint32_t)job.seed, k1 = (uint32_t)(job.seed >> 32);
-            for (uint32_t q = 0; q < kSamplesPerLane / per; ++q) {
+            for (uint32_t q = 0; q < spl / per; ++q) {
                 const uint64_t s = base + ((uint64_t)q * 64 + lane) * per;  // first local sample of this lane
                 const uint64_t blk = (job.first + s) / per;
                 uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), job.stream, kMcTag};
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
                 }
             }
         } else {
-            for (uint32_t q = 0; q < kSamplesPerLane; ++q) {
+            for (uint32_t q = 0; q < spl; ++q) {
                 const uint64_t s = base + (uint64_t)q * 64 + lane;
                 const bool ok = s < count;
                 uint64_t d;
@@ -205,7 +205,10 @@ cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_it
     return CR_OK;
 }
 
-uint64_t hamming_items(uint64_t count) { return (count + kItemSamples - 1) / kItemSamples; }
+uint64_t hamming_items(uint64_t count, uint32_t lane_samples) {
+    const uint64_t per_item = 64ull * lane_samples;
+    return (count + per_item - 1) / per_item;
+}
 
 uint32_t hamming_waves_per_cu(uint32_t lds_rows) {
     const size_t lds = (size_t)lds_rows * kStride * sizeof(uint32_t);

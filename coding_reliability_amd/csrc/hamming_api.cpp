@@ -1,4 +1,5 @@
 // Host side of the Extended Hamming and AN Monte Carlo entry points.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -62,7 +63,14 @@ extern "C" cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *r
     job.stream = req->stream;
     job.nbins = need;
     job.item_begin = 0;
-    job.items = hamming_items(req->count);
+    {
+        // items of up to 64 x kHamMaxLaneSamples words, at least ~4 per resident wave
+        const uint64_t waves = (uint64_t)hamming_waves_per_cu(need + 1) * ctx->cus;
+        uint64_t spl = (req->count + 64 * 4 * waves - 1) / (64 * 4 * waves);
+        spl = std::min<uint64_t>(std::max<uint64_t>((spl + 3) / 4 * 4, 16), kHamMaxLaneSamples);
+        job.lane_samples = (uint32_t)spl;
+    }
+    job.items = hamming_items(req->count, job.lane_samples);
     if (req->source == CR_HAM_LIST) {
         CR_TRY(ctx->table_a.reserve(req->count * sizeof(uint64_t)));
         CR_HIP(hipMemcpyAsync(ctx->table_a.ptr, req->list, req->count * sizeof(uint64_t), hipMemcpyHostToDevice,

@@ -129,8 +129,9 @@ cr_status describe(cr_server *s, const cr_job &j, Request &r) {
         h.seed = j.seed;
         h.stream = j.stream;
         h.nbins = j.k + rbits + 1;
+        h.lane_samples = kHamMaxLaneSamples;
         r.nbins = h.nbins;
-        r.total_items = hamming_items(j.samples);
+        r.total_items = hamming_items(j.samples, h.lane_samples);
         r.units = j.samples;
     } else {
         return fail(CR_ERR_INVALID_ARGUMENT, "unknown request kind %u", j.kind);
