@@ -20,8 +20,9 @@
 //      (no pruning: a prefix minimum proves nothing about the maximum);
 //   2. bound: the 64 A with the largest pm over the rest of the range: their
 //      exact minima give L, a minimum that some A attains, so max >= L;
-//   3. the A with pm >= L (the others cannot attain the maximum) over the rest
-//      of the range, compacted into full waves, with gbest starting at L.
+//   3. the A whose minimum so far is >= L (the others cannot attain the
+//      maximum) over the rest of the range in rounds of 8x growing d ranges,
+//      compacted into full waves before each round, with gbest at L.
 // Every A attaining the maximum survives each step and is scanned in full, so
 // the maximum and the set of A attaining it are exact.
 #include <algorithm>
@@ -41,6 +42,7 @@ constexpr uint64_t kCheck = 1u << 11;      // steps between early-exit checks
 constexpr uint64_t kMinChunk = 1u << 14;   // smallest d chunk worth an item
 constexpr uint64_t kPrefix = 1u << 12;     // d range of the prefix pass
 constexpr uint64_t kBoundA = 64;           // A scanned in full for the lower bound
+constexpr uint64_t kRoundGrowth = 8;       // d range growth per survivor round
 
 __global__ void cand_init_kernel(uint32_t *__restrict__ done, unsigned long long *__restrict__ ctr, uint64_t n,
                                  uint32_t gbest0) {
@@ -198,19 +200,25 @@ extern "C" cr_status cr_an_candidates(cr_context *ctx, uint32_t k, uint32_t abit
             v[order[i]] = lv[i];
             L = std::max(L, lv[i]);
         }
-        // 3. the remaining A whose prefix minimum reaches L
+        // 3. the remaining A whose minimum so far reaches L, over d ranges
+        //    growing 8x per round, recompacted between rounds
         std::vector<uint8_t> in_bound(numA, 0);
         for (uint64_t i = 0; i < nb; ++i) in_bound[order[i]] = 1;
-        std::vector<uint64_t> sa, si;
-        std::vector<uint32_t> sv;
-        for (uint64_t i = 0; i < numA; ++i)
-            if (!in_bound[i] && v[i] >= L) {
-                si.push_back(i);
-                sa.push_back(minA + 2 * i);
-                sv.push_back(v[i]);
-            }
-        CR_TRY(cand_scan(ctx, sa, 0, sa.size(), P, nwords, L, true, sv, &ms));
-        for (size_t j = 0; j < si.size(); ++j) v[si[j]] = sv[j];
+        for (uint64_t lo = P; lo < nwords;) {
+            const uint64_t hi = lo > nwords / kRoundGrowth ? nwords : lo * kRoundGrowth;
+            std::vector<uint64_t> sa, si;
+            std::vector<uint32_t> sv;
+            for (uint64_t i = 0; i < numA; ++i)
+                if (!in_bound[i] && v[i] >= L) {
+                    si.push_back(i);
+                    sa.push_back(minA + 2 * i);
+                    sv.push_back(v[i]);
+                }
+            if (si.empty()) break;
+            CR_TRY(cand_scan(ctx, sa, 0, sa.size(), lo, hi, L, hi == nwords, sv, &ms));
+            for (size_t j = 0; j < si.size(); ++j) v[si[j]] = sv[j];
+            lo = hi;
+        }
     }
     uint32_t b = 0;
     for (uint32_t x : v) b = x > b ? x : b;
