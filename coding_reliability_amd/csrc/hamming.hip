@@ -26,21 +26,39 @@ using namespace dev;
 // throughput limit (about 8e7 items/s, reached with 4096-word items at 3.4e11
 // words/s); small jobs use smaller items so that they still fill the device.
 
+// Weight of the non-extended codeword of data word d & dmask.  The parity
+// masks are already truncated to k bits, so they are applied to the raw word
+// (one AND with a scalar operand each; no per-mask re-masking).
 template <bool W64>
-__device__ __forceinline__ uint32_t weight_nonext(uint64_t d, const uint64_t (&m)[7]) {
+__device__ __forceinline__ uint32_t weight_nonext(uint64_t d, uint64_t dmask, const uint64_t (&m)[7]) {
     if (W64) {
-        uint32_t w = __popcll(d);
+        uint32_t w = __popcll(d & dmask);
 #pragma unroll
         for (int t = 0; t < 7; ++t) w += __popcll(d & m[t]) & 1u;
         return w;
     } else {
         const uint32_t x = (uint32_t)d;
-        uint32_t w = __popc(x);
+        uint32_t w = __popc(x & (uint32_t)dmask);
 #pragma unroll
         for (int t = 0; t < 6; ++t) w += __popc(x & (uint32_t)m[t]) & 1u;
         return w;
     }
 }
+
+// Syndrome of a 32-bit word: bit t = parity(x & m_t), t < 6.
+__device__ __forceinline__ uint32_t syndrome32(uint32_t x, const uint64_t (&m)[7]) {
+    uint32_t syn = 0;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) syn |= (__popc(x & (uint32_t)m[t]) & 1u) << t;
+    return syn;
+}
+
+// Exhaustive fast path: an aligned item holds the words d = dh | 64*q | lane
+// (disjoint bit fields), and both popcount and syndrome split over disjoint
+// fields (the syndrome is linear over GF(2)): weight(d) = popc(dh) + popc(x) +
+// popc(S(dh) ^ S(x)), x = 64*ql | lane, with q = 16*qh + ql.  S(x) and
+// popc(x) are per-lane constants for ql < 16; S(dh), popc(dh) are wave-uniform.
+constexpr uint32_t kFastUnroll = 16;
 
 __device__ __forceinline__ uint32_t find_job(const HamJob *jobs, uint32_t njobs, uint64_t t, uint32_t cur) {
     if (cur != 0xFFFFFFFFu && t >= jobs[cur].item_begin && t < jobs[cur].item_begin + jobs[cur].items) return cur;
@@ -62,6 +80,8 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
     const uint32_t dummy = lds_rows - 1;
     uint32_t cur = 0xFFFFFFFFu;
     uint64_t pending = 0;
+    uint32_t fast_job = 0xFFFFFFFFu;  // job whose per-lane constants are loaded
+    uint32_t fsyn[kFastUnroll], fbase[kFastUnroll];
 
     for (;;) {
         const uint64_t t = next_item(counter);
@@ -93,16 +113,38 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
                 if (per == 4) {
 #pragma unroll
                     for (uint32_t u = 0; u < 4; ++u) {
-                        const uint32_t w = weight_nonext<W64>((uint64_t)c[u] & dmask, m);
+                        const uint32_t w = weight_nonext<W64>((uint64_t)c[u], dmask, m);
                         lds_inc(lane_col + (s + u < count ? w : dummy) * kStride);
                     }
                 } else {
 #pragma unroll
                     for (uint32_t u = 0; u < 2; ++u) {
-                        const uint64_t d = ((uint64_t)c[2 * u] | ((uint64_t)c[2 * u + 1] << 32)) & dmask;
-                        const uint32_t w = weight_nonext<W64>(d, m);
+                        const uint64_t d = (uint64_t)c[2 * u] | ((uint64_t)c[2 * u + 1] << 32);
+                        const uint32_t w = weight_nonext<W64>(d, dmask, m);
                         lds_inc(lane_col + (s + u < count ? w : dummy) * kStride);
                     }
+                }
+            }
+        } else if (!W64 && job.source == kHamRange && base + 64ull * spl <= count && spl % kFastUnroll == 0 &&
+                   ((job.first + base) & (64ull * spl - 1)) == 0 && 64ull * spl - 1 <= dmask) {
+            // full aligned item of consecutive words (exhaustive)
+            if (fast_job != jb) {
+#pragma unroll
+                for (uint32_t ql = 0; ql < kFastUnroll; ++ql) {
+                    const uint32_t x = (ql << 6) | lane;
+                    fsyn[ql] = syndrome32(x, m);
+                    fbase[ql] = lane * 4 + ((uint32_t)__popc(x) << 8);  // byte offset in the LDS rows
+                }
+                fast_job = jb;
+            }
+            const uint32_t d0 = (uint32_t)((job.first + base) & dmask);
+            for (uint32_t qh = 0; qh < spl / kFastUnroll; ++qh) {
+                const uint32_t dh = uniform32(d0 | (qh << 10));
+                const uint32_t sh = uniform32(syndrome32(dh, m)), ph = uniform32(__popc(dh));
+#pragma unroll
+                for (uint32_t ql = 0; ql < kFastUnroll; ++ql) {
+                    const uint32_t w = __popc(fsyn[ql] ^ sh) + ph;
+                    lds_inc(reinterpret_cast<uint32_t *>(reinterpret_cast<char *>(lds) + ((w << 8) + fbase[ql])));
                 }
             }
         } else {
@@ -116,7 +158,7 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
                     d = job.first + s * job.stride;
                 else
                     d = ok ? job.list[s] : 0;
-                const uint32_t w = weight_nonext<W64>(d & dmask, m);
+                const uint32_t w = weight_nonext<W64>(d, dmask, m);
                 lds_inc(lane_col + (ok ? w : dummy) * kStride);
             }
         }
