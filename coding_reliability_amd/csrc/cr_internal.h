@@ -142,11 +142,11 @@ struct HamJob {
     unsigned long long *out;   // device weight histogram (nbins entries)
     uint64_t masks[7];         // parity masks truncated to k bits, zero when unused
     uint32_t source, k, stream, nbins;
-    uint32_t lane_samples;     // samples per lane per work item (multiple of 4, <= kHamMaxLaneSamples)
+    uint32_t lane_samples;     // samples per lane per work item (power of two in [4, kHamMaxLaneSamples])
     uint32_t pad;
 };
 static_assert(sizeof(HamJob) == 152, "HamJob layout");
-constexpr uint32_t kHamMaxLaneSamples = 512;
+constexpr uint32_t kHamMaxLaneSamples = 4096;
 
 // Parity masks of the (N,k) extended Hamming codes, truncated to k bits;
 // returns the number of non-overall parity bits.

@@ -21,10 +21,11 @@ namespace {
 
 using namespace dev;
 
-// Work item = 64 lanes x job.lane_samples samples.  Up to 32768 words per item
-// keep the work-queue counter (one device-scope atomic per item) far below its
+// Work item = 64 lanes x job.lane_samples samples.  Items of up to 2^18 words
+// keep the work-queue counter (one device-scope atomic per item) below its
 // throughput limit (about 8e7 items/s, reached with 4096-word items at 3.4e11
-// words/s); small jobs use smaller items so that they still fill the device.
+// words/s and with 32768-word items at 2.5e12 words/s on the exhaustive fast
+// path); small jobs use smaller items so that they still fill the device.
 
 // Weight of the non-extended codeword of data word d & dmask.  The parity
 // masks are already truncated to k bits, so they are applied to the raw word
@@ -125,9 +126,12 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
                     }
                 }
             }
-        } else if (!W64 && job.source == kHamRange && base + 64ull * spl <= count && spl % kFastUnroll == 0 &&
-                   ((job.first + base) & (64ull * spl - 1)) == 0 && 64ull * spl - 1 <= dmask) {
-            // full aligned item of consecutive words (exhaustive)
+        } else if (!W64 && job.source == kHamRange && base + 64ull * spl <= count && spl >= kFastUnroll &&
+                   (spl & (spl - 1)) == 0 && ((job.first + base) & (64ull * spl - 1)) == 0 &&
+                   64ull * spl - 1 <= dmask) {
+            // full item of consecutive words (exhaustive) whose first word is a
+            // multiple of the item size 64*spl, a power of two: the item's
+            // words are d0 | x for x < 64*spl (disjoint bits)
             if (fast_job != jb) {
 #pragma unroll
                 for (uint32_t ql = 0; ql < kFastUnroll; ++ql) {

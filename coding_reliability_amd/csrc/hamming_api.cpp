@@ -64,10 +64,12 @@ extern "C" cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *r
     job.nbins = need;
     job.item_begin = 0;
     {
-        // items of up to 64 x kHamMaxLaneSamples words, at least ~4 per resident wave
+        // items of 64 x spl words, spl a power of two in [16, kHamMaxLaneSamples],
+        // about 4 or more per resident wave
         const uint64_t waves = (uint64_t)hamming_waves_per_cu(need + 1) * ctx->cus;
-        uint64_t spl = (req->count + 64 * 4 * waves - 1) / (64 * 4 * waves);
-        spl = std::min<uint64_t>(std::max<uint64_t>((spl + 3) / 4 * 4, 16), kHamMaxLaneSamples);
+        const uint64_t want = (req->count + 64 * 4 * waves - 1) / (64 * 4 * waves);
+        uint64_t spl = 16;
+        while (spl < want && spl < kHamMaxLaneSamples) spl *= 2;
         job.lane_samples = (uint32_t)spl;
     }
     job.items = hamming_items(req->count, job.lane_samples);

@@ -27,10 +27,26 @@ def merged_even(h):
     return [(h[w] + (h[w - 1] if w else 0)) if w % 2 == 0 and w < len(h) else 0 for w in range(n)]
 
 
-@pytest.mark.parametrize("k", [4, 8, 12, 16, 20, 24])
+@pytest.mark.parametrize("k", [4, 8, 12, 16, 20, 24, 25, 26, 27, 28])
 def test_exhaustive_vs_oracle(ctx, k):
     h, _ = ctx.hamming_weights(k)
     assert h == O.hamming_weights(k)[: len(h)]
+
+
+@pytest.mark.parametrize("k,first,count", [(32, 0, 3 << 24), (32, 1 << 30, 5 << 22), (32, (1 << 32) - (1 << 24), 1 << 25),
+                                           (30, 12345 << 10, 1 << 26), (28, 1 << 20, (1 << 28) - (1 << 20))])
+def test_exhaustive_windows_vs_oracle(ctx, k, first, count):
+    # aligned and unaligned windows, wrap-around past 2^k, partial last items:
+    # the exhaustive fast path (aligned full items) and the generic path meet
+    h, _ = ctx.hamming_weights(k, cra.HAM_RANGE, first=first, count=count)
+    mask = (1 << k) - 1
+    lo, hi = first & mask, (first + count) & mask
+    if first + count <= (first & ~mask) + (1 << k):
+        ref = O.hamming_weights(k, lo, lo + count)
+    else:  # wraps: [lo, 2^k) and [0, hi)
+        a, b = O.hamming_weights(k, lo, 1 << k), O.hamming_weights(k, 0, hi)
+        ref = [x + y for x, y in zip(a, b)]
+    assert h == ref[: len(h)]
 
 
 @pytest.mark.parametrize("k,name", [(8, "sol_hamming_n8"), (16, "sol_hamming_n16"), (24, "sol_hamming_n24"),
