@@ -33,6 +33,7 @@ const int kOffset = 2;  // first bin of the error columns (globals.cpp:24)
 
 std::mutex g_mu;
 cr_context *g_ctx[64] = {};
+std::recursive_mutex g_dev_mu[64];  // one call at a time per device context
 
 [[noreturn]] void raise(const char *what) {
     throw std::runtime_error(std::string(what) + ": " + cr_last_error());
@@ -43,13 +44,26 @@ void check(cr_status s, const char *what) {
 }
 
 // One context per device for the whole process (the reference created and
-// reset the device on every call).
+// reset the device on every call).  A lease holds the device's call lock for
+// the duration of an entry point: the reference functions may be called from
+// several threads, and a context serves one call at a time.
 cr_context *context_for(int dev) {
     if (dev < 0 || dev >= 64) throw std::runtime_error("device index out of range");
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_ctx[dev]) check(cr_context_create(dev, &g_ctx[dev]), "cr_context_create");
     return g_ctx[dev];
 }
+
+std::recursive_mutex &device_mutex(int dev) {
+    if (dev < 0 || dev >= 64) throw std::runtime_error("device index out of range");
+    return g_dev_mu[dev];
+}
+
+struct Lease {
+    std::unique_lock<std::recursive_mutex> lock;
+    cr_context *ctx;
+    explicit Lease(int dev) : lock(device_mutex(dev)), ctx(context_for(dev)) {}
+};
 
 struct WallClock {
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -411,7 +425,8 @@ void run_ancoding(uintll_t n, uintll_t A, int verbose, double *times, uintll_t *
     if ((A & (A - 1)) == 0) A = 1;
     const uint_t h = ceil_log2(A);
     const int dev = device_of(nr_dev);
-    cr_context *ctx = context_for(dev);
+    Lease lease(dev);
+    cr_context *ctx = lease.ctx;
     const std::string app = device_line(dev);
     WallClock wc;
     const uint_t count_counts = (uint_t)n + h + 1;
@@ -439,7 +454,8 @@ double run_ancoding_grid(int gdim, uintll_t n, uintll_t iterations, uintll_t ite
     // an_coding_grid.cu:215-362
     if ((A & (A - 1)) == 0) A = 1;
     const int dev = device_of(nr_dev);
-    cr_context *ctx = context_for(dev);
+    Lease lease(dev);
+    cr_context *ctx = lease.ctx;
     if (verbose > 1)
         printf("Start AN-Coding Algorithm - %dD Grid with %llu x %llu points\n", gdim, iterations,
                gdim == 2 ? iterations2 : 1);
@@ -480,7 +496,8 @@ double run_ancoding_mc(uintll_t n, uintll_t iterations, uintll_t A, int verbose,
     // an_coding_mc.cu:205-343 with Philox samples (seed 16384, the reference's device-0 seed)
     if ((A & (A - 1)) == 0) A = 1;
     const int dev = device_of(nr_dev);
-    cr_context *ctx = context_for(dev);
+    Lease lease(dev);
+    cr_context *ctx = lease.ctx;
     WallClock wc;
     const uintll_t msgs = 1ull << n;
     const uint_t count_counts = (uint_t)n + ceil_log2(A) + 1;
@@ -509,7 +526,8 @@ double run_ancoding_mc(uintll_t n, uintll_t iterations, uintll_t A, int verbose,
 void run_hamming(uintll_t n, int with_1bit, int file_output, int nr_dev) {
     // hamming.cu:97-224: weights of all 2^n codewords, merge, spheres, << n
     const int dev = device_of(nr_dev);
-    cr_context *ctx = context_for(dev);
+    Lease lease(dev);
+    cr_context *ctx = lease.ctx;
     WallClock wc;
     const uint_t r = cr_hamming_parity_bits((uint32_t)n);
     const uint_t N = (uint_t)n + r + 1, h = r + 1;
@@ -534,7 +552,8 @@ void run_hamming(uintll_t n, int with_1bit, int file_output, int nr_dev) {
 double run_hamming_grid(uintll_t n, int with_1bit, uintll_t iterations, int file_output, int nr_dev) {
     // hamming_grid.cu:91-219: sample points x = shard*S*(u64)step + k*step
     const int dev = device_of(nr_dev);
-    cr_context *ctx = context_for(dev);
+    Lease lease(dev);
+    cr_context *ctx = lease.ctx;
     hipDeviceProp_t p;
     const char *name = hipGetDeviceProperties(&p, dev) == hipSuccess ? p.name : "";
     printf("Start 1D Hamming-Coding Algorithm - 1D Grid with %llu iterations\n", iterations);
@@ -578,7 +597,8 @@ double run_hamming_grid(uintll_t n, int with_1bit, uintll_t iterations, int file
 double run_hamming_mc(uintll_t n, int with_1bit, uintll_t iterations, int file_output, int nr_dev) {
     // hamming_mc.cu:98-231 with Philox words (seed 1337, the reference's device-0 seed)
     const int dev = device_of(nr_dev);
-    cr_context *ctx = context_for(dev);
+    Lease lease(dev);
+    cr_context *ctx = lease.ctx;
     hipDeviceProp_t p;
     const char *name = hipGetDeviceProperties(&p, dev) == hipSuccess ? p.name : "";
     printf("Start Hamming-Coding Algorithm - Monte Carlo with %llu iterations\n", iterations);
@@ -636,7 +656,8 @@ void run_ancoding(uintll_t A, uint_t h, Flags &flags, double *times, uint128_t *
     // super_a/src/an_coding.cu:135-244: counts returned to the caller, doubled in 64 bits
     if ((A & (A - 1)) == 0) A = 1;
     const int dev = flags.dev >= 0 ? flags.dev : 0;
-    cr_context *ctx = context_for(dev);
+    Lease lease(dev);
+    cr_context *ctx = lease.ctx;
     WallClock wc;
     const uint_t n = (uint_t)flags.n;
     const uint_t count_counts = n + h + 1;
@@ -656,7 +677,8 @@ void run_ancoding_grid(uintll_t A, uint_t h, Flags &flags, double *times, uint12
     // super_a/src/an_coding_grid.cu:179-289
     if ((A & (A - 1)) == 0) A = 1;
     const int dev = flags.dev >= 0 ? flags.dev : 0;
-    cr_context *ctx = context_for(dev);
+    Lease lease(dev);
+    cr_context *ctx = lease.ctx;
     WallClock wc;
     const uint_t n = (uint_t)flags.n;
     const uint_t count_counts = n + h + 1;

@@ -160,3 +160,44 @@ def test_more_k24_fixtures_batched(ctx):
     print(f"two k=24 codes: {ms:.1f} ms", flush=True)
     for r, raw in zip(runs, got):
         assert report(24, raw + [0] * 64, len(r["counts"])) == r["counts"], r["src"]
+
+
+@pytest.mark.parametrize("A,N", [((1 << 64) - 1, 2), ((1 << 63) + 1, 2), ((1 << 62) + 7, 4), ((1 << 32) - 1, 1 << 16),
+                                 (0x5555555555555555, 3)])
+def test_extreme_widths(ctx, A, N):
+    # codewords up to 64 bits (65 bins), a single pair, the widest words
+    raw, _ = ctx.an_pairs(A, N, nbins=65)
+    assert raw == oracle_raw(A, N, 65), (A, N)
+    assert sum(raw) == N * (N - 1) // 2
+
+
+def test_empty_and_single_word(ctx):
+    import coding_reliability_amd as cra
+    raw, _ = ctx.an_pairs(61, 1)
+    assert sum(raw) == 0
+    with pytest.raises(cra.CRError):
+        ctx.an_pairs(61, 0)
+
+
+def test_two_contexts_from_two_threads(ctx):
+    # contexts share no state: two threads, one context each, run at once
+    import threading
+
+    import coding_reliability_amd as cra
+    jobs = [(58659, 1 << 16), (61, 1 << 17)]
+    want = [ctx.an_pairs(A, N)[0] for A, N in jobs]
+    got = [None, None]
+
+    def run(i):
+        c = cra.Context(0)
+        try:
+            got[i] = [c.an_pairs(A, N)[0] for A, N in jobs[i:] + jobs[:i]][0]
+        finally:
+            c.close()
+
+    ts = [threading.Thread(target=run, args=(i,)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=120)
+    assert got == want
