@@ -31,6 +31,7 @@ A_HEADLINE = 58659
 K_HEADLINE = 24
 PEAK_CLOCK_GHZ = 2.4
 LANES_PER_CU_CLK = 64
+LDS_ATOMIC_LANES_PER_CU_CLK = 16
 
 
 def log(msg: str) -> None:
@@ -402,6 +403,15 @@ def main() -> int:
         "ops_per_pair": ops_per_pair,
         "survey_ops_per_pair_2W_plus_1": 2 * W + 1,
         "peak_clock_ghz": PEAK_CLOCK_GHZ,
+        # the binding limit of this kernel: one LDS increment per pair, and a
+        # wave-wide ds_add_u32 holds a CU's LDS for 4 clocks (16 lanes/clk,
+        # measured with SQ_INSTS_LDS / GRBM_GUI_ACTIVE on this device)
+        "lds_atomic": {
+            "achieved": round(pairs_per_step / avg_kernel_s / 1e12, 3),
+            "peak": round(cus * LDS_ATOMIC_LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12, 3),
+            "unit": "T increments/s",
+            "frac": round(pairs_per_step / avg_kernel_s / (cus * LDS_ATOMIC_LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9), 4),
+        },
     }
 
     batched = None
