@@ -21,6 +21,35 @@ namespace {
 
 using namespace dev;
 
+// Rows whose high dwords are all equal (h, wave-uniform): the high-dword
+// distance popcount(h ^ v_hi) is one scalar per column, folded into the
+// popcount of the low dwords as its accumulate operand (3 VALU per pair, as
+// for 32-bit words).  Generated rows A*i of one row block share their high
+// dword except where A*i crosses a multiple of 2^32.
+template <int R>
+__device__ __forceinline__ void rect_cols_hu(uint32_t *lane_col, const uint32_t (&wl)[R], uint32_t h,
+                                             const uint64_t *__restrict__ c, uint32_t n) {
+    uint32_t q = 0;
+    for (; q + 4 <= n; q += 4) {
+        uint64_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = uniform64(c[q + u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t vl = (uint32_t)v[u];
+            const uint32_t hd = uniform32((uint32_t)__popc(h ^ (uint32_t)(v[u] >> 32)));
+#pragma unroll
+            for (int r = 0; r < R; ++r) lds_inc(lane_col + (__popc(wl[r] ^ vl) + hd) * kStride);
+        }
+    }
+    for (; q < n; ++q) {
+        const uint64_t v = uniform64(c[q]);
+        const uint32_t hd = uniform32((uint32_t)__popc(h ^ (uint32_t)(v >> 32)));
+#pragma unroll
+        for (int r = 0; r < R; ++r) lds_inc(lane_col + (__popc(wl[r] ^ (uint32_t)v) + hd) * kStride);
+    }
+}
+
 template <int R, bool W2>
 __device__ __forceinline__ void rect_cols(uint32_t *lane_col, const uint64_t (&w)[R], const uint64_t *__restrict__ cols,
                                           uint64_t j0, uint32_t n) {
@@ -31,6 +60,16 @@ __device__ __forceinline__ void rect_cols(uint32_t *lane_col, const uint64_t (&w
         wh[r] = (uint32_t)(w[r] >> 32);
     }
     const uint64_t *c = cols + j0;
+    if (W2) {
+        const uint32_t h = uniform32(wh[0]);
+        bool same = true;
+#pragma unroll
+        for (int r = 0; r < R; ++r) same = same && wh[r] == h;
+        if (__all(same)) {
+            rect_cols_hu<R>(lane_col, wl, h, c, n);
+            return;
+        }
+    }
     uint32_t q = 0;
     for (; q + 4 <= n; q += 4) {
         uint64_t v[4];
