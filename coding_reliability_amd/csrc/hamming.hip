@@ -27,23 +27,43 @@ using namespace dev;
 // words/s and with 32768-word items at 2.5e12 words/s on the exhaustive fast
 // path); small jobs use smaller items so that they still fill the device.
 
-// Weight of the non-extended codeword of data word d & dmask.  The parity
-// masks are already truncated to k bits, so they are applied to the raw word
-// (one AND with a scalar operand each; no per-mask re-masking).
+// Syndrome tables: the syndrome (bit t = parity(d & m_t)) is linear over
+// GF(2), so it is the XOR of the syndromes of the data word's bytes; table j
+// holds the syndromes of the 256 values of byte j (4 tables for 32-bit data,
+// 8 for 64-bit).  The masks are truncated to k bits, so bits of d above k do
+// not change the syndrome.  The tables sit at the start of the wave's LDS,
+// the histogram rows after them.
 template <bool W64>
-__device__ __forceinline__ uint32_t weight_nonext(uint64_t d, uint64_t dmask, const uint64_t (&m)[7]) {
-    if (W64) {
-        uint32_t w = __popcll(d & dmask);
+constexpr uint32_t kTabBytes = W64 ? 8 * 256 : 4 * 256;
+
+template <bool W64>
+__device__ void build_tables(uint8_t *tab, const uint64_t *masks) {
+    uint64_t m[7];
 #pragma unroll
-        for (int t = 0; t < 7; ++t) w += __popcll(d & m[t]) & 1u;
-        return w;
-    } else {
-        const uint32_t x = (uint32_t)d;
-        uint32_t w = __popc(x & (uint32_t)dmask);
+    for (int t = 0; t < 7; ++t) m[t] = masks[t];
+    for (uint32_t e = threadIdx.x; e < kTabBytes<W64>; e += 64) {
+        const uint64_t x = (uint64_t)(e & 255) << (8 * (e >> 8));
+        uint32_t syn = 0;
 #pragma unroll
-        for (int t = 0; t < 6; ++t) w += __popc(x & (uint32_t)m[t]) & 1u;
-        return w;
+        for (int t = 0; t < 7; ++t) syn |= (uint32_t)(__popcll(x & m[t]) & 1) << t;
+        tab[e] = (uint8_t)syn;
     }
+    __syncthreads();
+}
+
+// Weight of the non-extended codeword of data word d & dmask: popcount of the
+// data plus popcount of the syndrome.
+template <bool W64>
+__device__ __forceinline__ uint32_t weight_tab(uint64_t d, uint64_t dmask, const uint8_t *tab) {
+    const uint32_t lo = (uint32_t)d;
+    uint32_t syn = tab[lo & 255] ^ tab[256 + ((lo >> 8) & 255)] ^ tab[512 + ((lo >> 16) & 255)] ^ tab[768 + (lo >> 24)];
+    if (W64) {
+        const uint32_t hi = (uint32_t)(d >> 32);
+        syn ^= tab[1024 + (hi & 255)] ^ tab[1280 + ((hi >> 8) & 255)] ^ tab[1536 + ((hi >> 16) & 255)] ^
+               tab[1792 + (hi >> 24)];
+        return __popc(syn) + __popcll(d & dmask);
+    }
+    return __popc(syn) + __popc(lo & (uint32_t)dmask);
 }
 
 // Syndrome of a 32-bit word: bit t = parity(x & m_t), t < 6.
@@ -61,6 +81,37 @@ __device__ __forceinline__ uint32_t syndrome32(uint32_t x, const uint64_t (&m)[7
 // popc(x) are per-lane constants for ql < 16; S(dh), popc(dh) are wave-uniform.
 constexpr uint32_t kFastUnroll = 16;
 
+// One Monte Carlo item: lane l draws the Philox blocks of its samples (4
+// words per block for k <= 32, 2 for k > 32).  CHECK: the item reaches past
+// the job's sample count, so samples >= count go to the dummy row.
+template <bool W64, bool CHECK>
+__device__ __forceinline__ void mc_item(uint32_t *lane_col, const uint8_t *tab, const HamJob &job, uint64_t base,
+                                        uint32_t spl, uint64_t count, uint64_t dmask, uint32_t dummy) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t per = job.k <= 32 ? 4 : 2;  // words per Philox block
+    const uint32_t k0 = (uint32_t)job.seed, k1 = (uint32_t)(job.seed >> 32);
+    for (uint32_t q = 0; q < spl / per; ++q) {
+        const uint64_t s = base + ((uint64_t)q * 64 + lane) * per;  // first local sample of this lane
+        const uint64_t blk = (job.first + s) / per;
+        uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), job.stream, kMcTag};
+        philox4x32_10(c, k0, k1);
+        if (per == 4) {
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const uint32_t w = weight_tab<W64>((uint64_t)c[u], dmask, tab);
+                lds_inc(lane_col + ((!CHECK || s + u < count) ? w : dummy) * kStride);
+            }
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < 2; ++u) {
+                const uint64_t d = (uint64_t)c[2 * u] | ((uint64_t)c[2 * u + 1] << 32);
+                const uint32_t w = weight_tab<W64>(d, dmask, tab);
+                lds_inc(lane_col + ((!CHECK || s + u < count) ? w : dummy) * kStride);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ uint32_t find_job(const HamJob *jobs, uint32_t njobs, uint64_t t, uint32_t cur) {
     if (cur != 0xFFFFFFFFu && t >= jobs[cur].item_begin && t < jobs[cur].item_begin + jobs[cur].items) return cur;
     uint32_t lo = 0, hi = njobs - 1;
@@ -74,7 +125,9 @@ __device__ __forceinline__ uint32_t find_job(const HamJob *jobs, uint32_t njobs,
 template <bool W64>
 __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs, uint32_t njobs, uint64_t total_items,
                                                  unsigned long long *__restrict__ counter, uint32_t lds_rows) {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ uint32_t lds_all[];
+    uint8_t *const tab = reinterpret_cast<uint8_t *>(lds_all);
+    uint32_t *const lds = lds_all + kTabBytes<W64> / 4;
     const uint32_t lane = threadIdx.x;
     lds_clear(lds, lds_rows);
     uint32_t *const lane_col = lds + lane;
@@ -92,6 +145,7 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
         const uint32_t spl = job.lane_samples;
         if (jb != cur || pending + spl >= 0xFFFFFFFFull) {
             if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, jobs[cur].out);
+            if (jb != cur) build_tables<W64>(tab, job.masks);
             cur = jb;
             pending = 0;
         }
@@ -104,28 +158,10 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
         const uint64_t count = job.count;
 
         if (job.source == kHamMc) {
-            const uint32_t per = job.k <= 32 ? 4 : 2;  // words per Philox block
-            const uint32_t k0 = (uint32_t)job.seed, k1 = (uint32_t)(job.seed >> 32);
-            for (uint32_t q = 0; q < spl / per; ++q) {
-                const uint64_t s = base + ((uint64_t)q * 64 + lane) * per;  // first local sample of this lane
-                const uint64_t blk = (job.first + s) / per;
-                uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), job.stream, kMcTag};
-                philox4x32_10(c, k0, k1);
-                if (per == 4) {
-#pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u) {
-                        const uint32_t w = weight_nonext<W64>((uint64_t)c[u], dmask, m);
-                        lds_inc(lane_col + (s + u < count ? w : dummy) * kStride);
-                    }
-                } else {
-#pragma unroll
-                    for (uint32_t u = 0; u < 2; ++u) {
-                        const uint64_t d = (uint64_t)c[2 * u] | ((uint64_t)c[2 * u + 1] << 32);
-                        const uint32_t w = weight_nonext<W64>(d, dmask, m);
-                        lds_inc(lane_col + (s + u < count ? w : dummy) * kStride);
-                    }
-                }
-            }
+            if (base + 64ull * spl <= count)
+                mc_item<W64, false>(lane_col, tab, job, base, spl, count, dmask, dummy);
+            else
+                mc_item<W64, true>(lane_col, tab, job, base, spl, count, dmask, dummy);
         } else if (!W64 && job.source == kHamRange && base + 64ull * spl <= count && spl >= kFastUnroll &&
                    (spl & (spl - 1)) == 0 && ((job.first + base) & (64ull * spl - 1)) == 0 &&
                    64ull * spl - 1 <= dmask) {
@@ -162,7 +198,7 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
                     d = job.first + s * job.stride;
                 else
                     d = ok ? job.list[s] : 0;
-                const uint32_t w = weight_nonext<W64>(d, dmask, m);
+                const uint32_t w = weight_tab<W64>(d, dmask, tab);
                 lds_inc(lane_col + (ok ? w : dummy) * kStride);
             }
         }
@@ -224,6 +260,11 @@ __global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint6
     if (dirty) flush_rows(lds, 0, nbins, out);
 }
 
+// Dynamic LDS of a Hamming wave: syndrome tables, then the histogram rows.
+size_t ham_lds_bytes(bool w64, uint32_t lds_rows) {
+    return (w64 ? kTabBytes<true> : kTabBytes<false>) + (size_t)lds_rows * kStride * sizeof(uint32_t);
+}
+
 }  // namespace
 
 cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_items, bool w64,
@@ -232,7 +273,7 @@ cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_it
     const void *k = w64 ? reinterpret_cast<const void *>(&ham_kernel<true>)
                         : reinterpret_cast<const void *>(&ham_kernel<false>);
     if (lds_rows < 2 || lds_rows > 131) return fail(CR_ERR_INVALID_ARGUMENT, "bad histogram rows %u", lds_rows);
-    const size_t lds = (size_t)lds_rows * kStride * sizeof(uint32_t);
+    const size_t lds = ham_lds_bytes(w64, lds_rows);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, lds) != hipSuccess || per_cu <= 0)
         return fail(CR_ERR_DEVICE, "occupancy query failed for the Hamming kernel");
@@ -257,13 +298,12 @@ uint64_t hamming_items(uint64_t count, uint32_t lane_samples) {
 }
 
 uint32_t hamming_waves_per_cu(uint32_t lds_rows) {
-    const size_t lds = (size_t)lds_rows * kStride * sizeof(uint32_t);
     int a = 0, b = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, reinterpret_cast<const void *>(&ham_kernel<false>), 64,
-                                                     lds) != hipSuccess)
+                                                     ham_lds_bytes(false, lds_rows)) != hipSuccess)
         a = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, reinterpret_cast<const void *>(&ham_kernel<true>), 64,
-                                                     lds) != hipSuccess)
+                                                     ham_lds_bytes(true, lds_rows)) != hipSuccess)
         b = 0;
     return (uint32_t)std::max(std::max(a, b), 1);
 }
