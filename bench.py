@@ -161,16 +161,17 @@ def run_batched(ctx, autotune: bool = True) -> dict:
 
 
 # Algorithmic int32 lane-ops per unit of the other kernels (VALU-bound all):
-#   Hamming weight of a sampled word: and + popcount per parity mask and data
-#   dword (2 r W), popcount of the data (W), one increment; Monte Carlo adds
-#   the Philox4x32-10 share (10 rounds x 8 ops per 4 words: 20 per 32-bit
-#   word, 40 per 64-bit word).  Exhaustive enumeration: the syndrome and the
+#   Hamming weight of a sampled word: one table lookup per data byte (the
+#   byte extract; the syndrome is linear, so it is the XOR of per-byte
+#   syndromes), the XORs, popcounts of syndrome and data, one increment: 9 for
+#   32-bit words, 16 for 64-bit words; Monte Carlo adds the Philox4x32-10
+#   share (10 rounds x 6 ops per block: 15 per 32-bit word, 30 per 64-bit).  Exhaustive enumeration: the syndrome and the
 #   popcount split into a per-lane and a per-iteration part (the words of an
 #   aligned block are disjoint bit fields), leaving xor, popcount-accumulate
 #   and the increment (3) per word.  Reference grid pair: xor + popcount and an
 #   increment (3; the rows of a row block share their high dword, so its
 #   distance is one scalar per sample, folded into the popcount).  AN Monte
-#   Carlo pair: 2W + 1, the multiply, and the Philox share (20).  ANcandidates step (A, d): c += A, 3c, xor, popcount on
+#   Carlo pair: 2W + 1, the multiply, and the Philox share (15).  ANcandidates step (A, d): c += A, 3c, xor, popcount on
 #   64 bits (8) and a min (1).
 def _kernel_entry(units: float, ms: float, ops: float, unit: str, peak: float, workload: str) -> dict:
     rate = units / (ms / 1e3)
@@ -191,16 +192,16 @@ def run_kernels(ctx, peak: float) -> dict:
                                                     "(39,32) weights of all 2^32 data words")
     h, ms = ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 20, stride=65537)
     assert sum(h) == 1 << 20
-    out["hamming_grid_39_32"] = _kernel_entry(1 << 20, ms, 14, "Gwords/s", peak,
+    out["hamming_grid_39_32"] = _kernel_entry(1 << 20, ms, 9, "Gwords/s", peak,
                                               "(39,32) integer grid d = i*65537 mod 2^32, i < 2^20")
     S = 1 << 34
     h, ms = ctx.hamming_weights(k, cra.HAM_MC, 0, S, seed=2026, stream=0)
     assert sum(h) == S
-    out["hamming_mc_39_32"] = _kernel_entry(S, ms, 34, "Gwords/s", peak, "(39,32) 2^34 Philox words")
+    out["hamming_mc_39_32"] = _kernel_entry(S, ms, 24, "Gwords/s", peak, "(39,32) 2^34 Philox words")
     S = 1 << 30
     h, ms = ctx.hamming_weights(64, cra.HAM_MC, 0, S, seed=2026, stream=1)
     assert sum(h) == S
-    out["hamming_mc_72_64"] = _kernel_entry(S, ms, 71, "Gwords/s", peak, "(72,64) 2^30 Philox words")
+    out["hamming_mc_72_64"] = _kernel_entry(S, ms, 46, "Gwords/s", peak, "(72,64) 2^30 Philox words")
     h, ms = ctx.an_grid_ref(1, 32, 1001, 0, 61)
     pairs = sum(h)
     out["an_grid1d_ref"] = _kernel_entry(pairs, ms, 3, "Gpairs/s", peak,
@@ -208,7 +209,7 @@ def run_kernels(ctx, peak: float) -> dict:
     m = 1 << 16
     h, ms = ctx.an_mc(61, 24, m, 16384)
     assert sum(h) == (1 << 24) * m
-    out["an_mc"] = _kernel_entry((1 << 24) * m, ms, 24, "Gpairs/s", peak, "AN Monte Carlo, A=61, k=24, 2^16 samples")
+    out["an_mc"] = _kernel_entry((1 << 24) * m, ms, 19, "Gpairs/s", peak, "AN Monte Carlo, A=61, k=24, 2^16 samples")
     best, cands, ms = ctx.an_candidates(24, 16)
     # pruning skips most (A, d) steps, so the work is counted as the steps the
     # candidates (A attaining the maximum, never pruned) must run in full: a
