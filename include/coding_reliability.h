@@ -73,7 +73,10 @@ cr_status cr_device_query(cr_context *ctx, cr_device_info *out);
 cr_status cr_an_pairs(cr_context *ctx, uint64_t A, uint64_t num_words, uint64_t *raw, uint32_t nbins,
                       double *kernel_ms);
 
-/* Tuning knobs for the pair kernel; 0 = automatic (autotuned table). */
+/* Tuning knobs for the pair kernel; 0 = automatic: 8 rows per lane from 2^16
+ * words (4 from 2^11, else 2) and column chunks giving about 16 items per
+ * resident wave (the settings measured fastest for single codes; batches are
+ * tuned by cr_autotune_batch). */
 typedef struct cr_pair_tuning {
     uint32_t rows_per_lane;   /* codewords held in registers per lane: 2, 4, 8 or 16 */
     uint32_t chunk_blocks;    /* column chunk in units of the 64*rows row block (single code) */
