@@ -223,6 +223,51 @@ def run_kernels(ctx, peak: float) -> dict:
     return out
 
 
+# Timings the reference records in its own result files (distance_distribution/
+# results/...), one K80-class GPU unless stated: (seconds, source).
+REFERENCE_RECORDED = {
+    "an_A61_k24": (1306.17, "runtimes/4/ancoding_gpu_A61_n24_h6.csv:34"),
+    "an_A32771_k24": (8058.97, "As/ancoding_gpu_A32771_n24_h16.csv:44"),
+    "grid1d_A61_k32_m1001": (107.01, "runtimes/4/ancoding_grid1d_gpu_m1001_A61_n32_h6.csv:42"),
+    "supera_k16_sweep": (1933.3, "superAs/superA_n16_h16.txt:37 (sum of 16384 per-code kernel times)"),
+    "supera_k8_sweep": (34.14, "superAs/superA_n8_h16.txt:34"),
+}
+
+
+def run_vs_reference(ctx) -> dict:
+    """The workloads the reference records timings for, on this device, each
+    checked (complete histograms; batched = per-code results)."""
+    from coding_reliability_amd import workloads as WL
+    out = {}
+
+    def entry(key, secs, extra=None):
+        ref, src = REFERENCE_RECORDED[key]
+        e = {"seconds": round(secs, 4), "reference_seconds": ref, "reference_source": src,
+             "speedup_vs_reference": round(ref / secs, 1)}
+        e.update(extra or {})
+        out[key] = e
+        log(f"{key}: {secs:.3f} s (reference {ref} s)")
+
+    for A, key in ((61, "an_A61_k24"), (32771, "an_A32771_k24")):
+        N = 1 << 24
+        raw, ms = ctx.an_pairs(A, N)
+        assert sum(raw) == N * (N - 1) // 2
+        entry(key, ms / 1e3, {"gpairs_per_s": round(N * (N - 1) / 2 / (ms / 1e3) / 1e9, 1)})
+    h, ms = ctx.an_grid_ref(1, 32, 1001, 0, 61)
+    entry("grid1d_A61_k32_m1001", ms / 1e3, {"gpairs_per_s": round(sum(h) / (ms / 1e3) / 1e9, 1)})
+    for k, key in ((16, "supera_k16_sweep"), (8, "supera_k8_sweep")):
+        sweep = WL.supera_codes(k, 16)
+        plan = ctx.batch_plan(sweep)
+        plan.run()
+        t0 = time.perf_counter()
+        res, _ = plan.run()
+        dt = time.perf_counter() - t0
+        plan.close()
+        assert all(sum(r) == N * (N - 1) // 2 for r, (A, N) in zip(res, sweep))
+        entry(key, dt, {"codes": len(sweep), "gpairs_per_s": round(WL.pairs_of(sweep) / dt / 1e9, 1)})
+    return out
+
+
 def run_served(ctx, njobs: int, rate: float, seed: int = 7, max_wait_s: float = 300.0) -> dict:
     """C5: Poisson arrivals (rate jobs/s) of a uniform mix of exhaustive AN
     (k in 18..24, odd 16-bit A), integer-grid AN (k=32, stride 4097, odd 16-bit
@@ -338,6 +383,8 @@ def main() -> int:
     ap.add_argument("--served", action="store_true", help="also run the served request mix")
     ap.add_argument("--no-kernels", action="store_true", help="skip the other kernels' measurements")
     ap.add_argument("--kernels-only", action="store_true", help="only the other kernels (for counter runs)")
+    ap.add_argument("--vs-reference", action="store_true",
+                    help="only the workloads the reference records timings for")
     ap.add_argument("--served-jobs", type=int, default=24)
     ap.add_argument("--served-rate", type=float, default=1.0)
     ap.add_argument("--traffic-probe", action="store_true", help="copy + one pair launch, for PMC counter runs")
@@ -362,6 +409,10 @@ def main() -> int:
     peak_tops = info["compute_units"] * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12
     if args.kernels_only:
         print(json.dumps({"kernels": run_kernels(ctx, peak_tops)}), flush=True)
+        ctx.close()
+        return 0
+    if args.vs_reference:
+        print(json.dumps({"vs_reference": run_vs_reference(ctx)}), flush=True)
         ctx.close()
         return 0
 
