@@ -45,3 +45,29 @@ def test_status_strings_without_gpu():
     lib.cr_status_string.restype = ctypes.c_char_p
     assert lib.cr_status_string(0) == b"ok"
     assert lib.cr_status_string(6) == b"admission refused"
+
+
+def test_null_arguments_are_refused_without_gpu():
+    # argument checks come before any device call: status 1, message set
+    from coding_reliability_amd import _lib
+    L = _lib.lib()
+    null = None
+    assert L.cr_an_pairs(null, 61, 16, null, 0, null) == 1
+    assert L.cr_an_batch(null, null, 0, null, 0, null, null) == 1
+    assert L.cr_hamming_weights(null, null, null, 0, null) == 1
+    assert L.cr_an_candidates(null, 8, 5, null, null, 0, null, null) == 1
+    assert L.cr_server_create(null, null, null) == 1
+    assert L.cr_batch_plan_create(null, null, 0, 0, null, null) == 1
+    assert L.cr_last_error() == b"NULL argument"
+    assert L.cr_hamming_report(8, null, 1, null) == 1
+
+
+def test_context_without_device_is_an_error():
+    from coding_reliability_amd import _lib
+    L = _lib.lib()
+    h = ctypes.c_void_p()
+    s = L.cr_context_create(0, ctypes.byref(h))
+    if s == 0:  # a GPU is present here
+        L.cr_context_destroy(h)
+        pytest.skip("a device is visible")
+    assert s in (3, 5) and L.cr_last_error()
