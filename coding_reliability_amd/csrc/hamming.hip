@@ -213,6 +213,32 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
 constexpr int RM = 4;
 constexpr uint32_t CJ = 256;  // samples per item (multiple of 4)
 
+// One AN Monte Carlo item.  W1: every codeword fits 32 bits (32-bit multiply,
+// xor and popcount).  CHECK: rows past 2^n or a partial sample chunk go to the
+// dummy row.
+template <bool W1, bool CHECK>
+__device__ __forceinline__ void an_mc_item(uint32_t *lane_col, uint64_t A, uint64_t rb, uint64_t nwords, uint64_t j0,
+                                           uint64_t j1, uint32_t dmask, uint32_t k0, uint32_t k1, uint32_t dummy) {
+    const uint32_t lane = threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < RM; ++r) {
+        const uint64_t i = rb * (64ull * RM) + (uint64_t)r * 64 + lane;
+        const bool ok = !CHECK || i < nwords;
+        const uint64_t w = A * i;
+        for (uint64_t j = j0; j < j1; j += 4) {
+            const uint64_t blk = j >> 2;
+            uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)i, kMcTag};
+            philox4x32_10(c, k0, k1);
+#pragma unroll
+            for (uint32_t u = 0; u < 4; ++u) {
+                const uint32_t d = W1 ? (uint32_t)__popc((uint32_t)w ^ ((uint32_t)A * (c[u] & dmask)))
+                                      : (uint32_t)__popcll(w ^ (A * (uint64_t)(c[u] & dmask)));
+                lds_inc(lane_col + ((!CHECK || (ok && j + u < j1)) ? d : dummy) * kStride);
+            }
+        }
+    }
+}
+
 __global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint64_t RBlocks,
                                                    uint64_t SC, uint64_t total_items,
                                                    unsigned long long *__restrict__ counter,
@@ -226,6 +252,8 @@ __global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint6
     const uint64_t nwords = 1ull << n;
     const uint32_t dmask = (uint32_t)(nwords - 1);
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    // all codewords (A * (2^n - 1)) below 2^32
+    const bool w1 = ((unsigned __int128)A * (nwords - 1)) >> 32 == 0;
     uint64_t pending = 0;
     bool dirty = false;
     for (;;) {
@@ -240,22 +268,13 @@ __global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint6
         }
         pending += j1 - j0;
         dirty = true;
-#pragma unroll
-        for (int r = 0; r < RM; ++r) {
-            const uint64_t i = rb * (64ull * RM) + (uint64_t)r * 64 + lane;
-            const bool ok = i < nwords;
-            const uint64_t w = A * i;
-            for (uint64_t j = j0; j < j1; j += 4) {
-                const uint64_t blk = j >> 2;
-                uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)i, kMcTag};
-                philox4x32_10(c, k0, k1);
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) {
-                    const uint32_t d = (uint32_t)__popcll(w ^ (A * (uint64_t)(c[u] & dmask)));
-                    lds_inc(lane_col + ((ok && j + u < j1) ? d : dummy) * kStride);
-                }
-            }
-        }
+        const bool full = (rb + 1) * (64ull * RM) <= nwords && j1 - j0 == CJ;
+        if (w1 && full)
+            an_mc_item<true, false>(lane_col, A, rb, nwords, j0, j1, dmask, k0, k1, dummy);
+        else if (full)
+            an_mc_item<false, false>(lane_col, A, rb, nwords, j0, j1, dmask, k0, k1, dummy);
+        else
+            an_mc_item<false, true>(lane_col, A, rb, nwords, j0, j1, dmask, k0, k1, dummy);
     }
     if (dirty) flush_rows(lds, 0, nbins, out);
 }
