@@ -330,6 +330,35 @@ def run_served(ctx, njobs: int, rate: float, seed: int = 7, max_wait_s: float = 
     }
 
 
+def run_served_small(ctx, njobs: int = 2048, seed: int = 11) -> dict:
+    """A burst of small exhaustive requests (k uniform in 8..14, odd 16-bit A)
+    through the request loop, against the same requests one per launch; every
+    served result equals its one-per-launch result."""
+    import numpy as np
+    import coding_reliability_amd as cra
+    g = np.random.Generator(np.random.Philox(seed))
+    jobs = [(int(2 * g.integers(1 << 14, 1 << 15) + 1), int(g.integers(8, 15))) for _ in range(njobs)]
+    t0 = time.perf_counter()
+    per = [ctx.an_pairs(A, 1 << k)[0] for A, k in jobs]
+    t_per = time.perf_counter() - t0
+    srv = cra.Server(ctx, iteration_ms=2, max_active=256)
+    t0 = time.perf_counter()
+    ids = [srv.submit(kind=cra.JOB_AN_EXHAUSTIVE, k=k, A=A) for A, k in jobs]
+    for rid in ids:
+        srv.wait(rid, timeout_ms=60000)
+    t_srv = time.perf_counter() - t0
+    got = [srv.result(rid)[0] for rid in ids]
+    st = srv.stats()
+    srv.close()
+    assert all(g_[: len(p)] == p[: len(g_)] for g_, p in zip(got, per)), "served result differs"
+    pairs = sum((1 << k) * ((1 << k) - 1) // 2 for _, k in jobs)
+    log(f"served small burst: {t_srv:.3f} s vs one per launch {t_per:.3f} s")
+    return {"workload": f"{njobs} exhaustive AN requests at once, k uniform in 8..14, odd 16-bit A, seed {seed}",
+            "served_s": round(t_srv, 4), "per_launch_s": round(t_per, 4),
+            "served_gpairs_per_s": round(pairs / t_srv / 1e9, 2), "per_launch_gpairs_per_s": round(pairs / t_per / 1e9, 2),
+            "speedup_served_vs_per_launch": round(t_per / t_srv, 1), "iterations": st["iterations"], "bit_exact": True}
+
+
 def traffic_probe(A: int, k: int) -> int:
     """For rocprofv3 --pmc runs: one device copy of exactly 1 GiB (torch, to
     scale FETCH_SIZE / WRITE_SIZE into bytes) followed by one pair-kernel
@@ -483,6 +512,7 @@ def main() -> int:
     if args.served:
         log("served mix ...")
         served = run_served(ctx, args.served_jobs, args.served_rate)
+        served["small_burst"] = run_served_small(ctx)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -323,17 +323,20 @@ void cr_server::loop() {
                 }
             }
         }
-        for (Request *r : fin) {
+        // copy every finished histogram back, then synchronise once
+        if (s == CR_OK && !fin.empty()) {
+            hipError_t e = hipSuccess;
+            for (size_t f = 0; f < fin.size() && e == hipSuccess; ++f)
+                e = hipMemcpyAsync(readback.as<uint64_t>() + f * kSlotBins,
+                                   slab.as<unsigned long long>() + (size_t)fin[f]->slot * kSlotBins,
+                                   fin[f]->nbins * sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) s = fail(CR_ERR_DEVICE, "result copy failed: %s", hipGetErrorString(e));
+        }
+        for (size_t f = 0; f < fin.size(); ++f) {
+            Request *r = fin[f];
             std::vector<uint64_t> h(r->nbins, 0);
-            if (s == CR_OK) {
-                hipError_t e = hipMemcpyAsync(readback.ptr, slab.as<unsigned long long>() + (size_t)r->slot * kSlotBins,
-                                              r->nbins * sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
-                if (e == hipSuccess) e = hipStreamSynchronize(stream);
-                if (e == hipSuccess)
-                    memcpy(h.data(), readback.ptr, r->nbins * sizeof(uint64_t));
-                else
-                    s = fail(CR_ERR_DEVICE, "result copy failed: %s", hipGetErrorString(e));
-            }
+            if (s == CR_OK) memcpy(h.data(), readback.as<uint64_t>() + f * kSlotBins, r->nbins * sizeof(uint64_t));
             std::lock_guard<std::mutex> lk(mu);
             r->result.swap(h);
             r->status = s;
@@ -369,7 +372,7 @@ cr_status cr_server_create(cr_context *ctx, const cr_server_config *cfg, cr_serv
     CR_HIP(hipEventCreate(&s->ev1));
     CR_TRY(s->slab.reserve((size_t)s->cfg.max_active * kSlotBins * sizeof(uint64_t)));
     CR_TRY(s->counter.reserve(4 * sizeof(uint64_t)));
-    CR_TRY(s->readback.reserve(kSlotBins * sizeof(uint64_t)));
+    CR_TRY(s->readback.reserve((size_t)s->cfg.max_active * kSlotBins * sizeof(uint64_t)));
     for (int i = (int)s->cfg.max_active - 1; i >= 0; --i) s->free_slots.push_back(i);
     s->t0 = std::chrono::steady_clock::now();
     cr_server *raw = s.release();

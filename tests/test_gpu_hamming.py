@@ -124,7 +124,8 @@ def test_mc_72_64_within_4sigma_macwilliams(ctx):
         assert abs(E[w] - S * p) <= 4 * math.sqrt(S * p * (1 - p)) + 1, w
 
 
-@pytest.mark.parametrize("A,n,m", [(61, 8, 16), (641, 10, 100), (58659, 12, 64), (3, 16, 5)])
+@pytest.mark.parametrize("A,n,m", [(61, 8, 16), (641, 10, 100), (58659, 12, 64), (3, 16, 5),
+                                   (61, 12, 1024), (58659, 16, 512), (641, 14, 700), (65533, 9, 1280)])
 def test_an_mc_bitexact_vs_oracle(ctx, A, n, m):
     h, _ = ctx.an_mc(A, n, m, seed=1234)
     assert h == O.an_mc(A, n, m, 1234)[: len(h)]

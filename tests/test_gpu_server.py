@@ -131,3 +131,21 @@ def test_large_request_runs_at_device_rate(ctx):
         got, info = srv.result(rid)
     assert got == ref[: len(got)]
     assert dt < 3 * kms / 1e3 + 1.0, (dt, kms, info)
+
+
+def test_burst_of_small_requests(ctx):
+    # many short requests at once: each result equals its one-shot launch,
+    # finished requests retire together (one copy-back synchronisation per iteration)
+    import random
+    rnd = random.Random(3)
+    jobs = [(2 * rnd.randrange(1 << 14, 1 << 15) + 1, rnd.randrange(6, 13)) for _ in range(300)]
+    with cra.Server(ctx, iteration_ms=2, max_active=128) as srv:
+        ids = [srv.submit(kind=cra.JOB_AN_EXHAUSTIVE, k=k, A=A) for A, k in jobs]
+        for rid in ids:
+            srv.wait(rid, timeout_ms=60000)
+        got = [srv.result(rid)[0] for rid in ids]
+        st = srv.stats()
+    for (A, k), g in zip(jobs, got):
+        ref = ctx.an_pairs(A, 1 << k)[0]
+        assert g == ref[: len(g)], (A, k)
+    assert st["completed"] == len(jobs)
