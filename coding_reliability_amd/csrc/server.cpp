@@ -110,9 +110,14 @@ cr_status describe(cr_server *s, const cr_job &j, Request &r) {
         p.N = N;
         p.mode = pair_mode(A, N);
         p.nbins = pair_max_distance(A, N) + 1;
-        // items of about 2^24 pairs (a few ms for one wave)
+        // items of about 1/64 of the request (at most 2^24 pairs, a few ms for
+        // one wave; at least one diagonal block): a small request spreads over
+        // many waves instead of waiting on a few long items
         const uint64_t RB = 64ull * R;
-        uint64_t Q = std::max<uint64_t>((1ull << 24) / (RB * RB), 1);
+        const long double pairs = (long double)N * (long double)(N - 1) / 2;
+        const long double per_item = std::min<long double>(std::max<long double>(pairs / 64, (long double)RB * RB),
+                                                            (long double)(1ull << 24));
+        uint64_t Q = std::max<uint64_t>((uint64_t)(per_item / ((long double)RB * RB)), 1);
         pair_job_geometry(p, R, (uint32_t)Q);
         r.nbins = p.nbins;
         r.total_items = p.items;
