@@ -35,6 +35,14 @@ namespace {
 
 constexpr uint32_t kSlotBins = 130;  // histogram slot size (uint64 counters)
 
+// Clears the listed histogram slots (one launch for all slots freed since the
+// last iteration; a slot is clean before its next request's first pass).
+__global__ void zero_slots_kernel(unsigned long long *__restrict__ slab, const uint32_t *__restrict__ slots,
+                                  uint32_t n) {
+    for (uint32_t b = blockIdx.x; b < n; b += gridDim.x)
+        for (uint32_t i = threadIdx.x; i < kSlotBins; i += blockDim.x) slab[(size_t)slots[b] * kSlotBins + i] = 0;
+}
+
 struct Request {
     uint64_t id = 0;
     cr_job job{};
@@ -62,9 +70,10 @@ struct cr_server {
     cr_server_config cfg{};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
-    DeviceBuffer slab, pair_jobs, ham_jobs, counter;
+    DeviceBuffer slab, pair_jobs, ham_jobs, counter, slot_list;
     HostBuffer staging, readback;
     std::vector<int> free_slots;
+    std::vector<uint32_t> dirty_slots;  // freed since the last iteration, cleared by it
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<uint64_t> queue;
@@ -212,7 +221,13 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
         return t;
     };
     const uint64_t t32 = rebase(h32), t64 = rebase(h64);
-    const size_t bytes = pjobs.size() * sizeof(PairJob) + (h32.size() + h64.size()) * sizeof(HamJob);
+    std::vector<uint32_t> zero;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        zero.swap(dirty_slots);
+    }
+    const size_t jbytes = pjobs.size() * sizeof(PairJob) + (h32.size() + h64.size()) * sizeof(HamJob);
+    const size_t bytes = jbytes + zero.size() * sizeof(uint32_t);
     CR_TRY(staging.reserve(bytes));
     CR_TRY(pair_jobs.reserve(std::max<size_t>(pjobs.size(), 1) * sizeof(PairJob)));
     CR_TRY(ham_jobs.reserve(std::max<size_t>(h32.size() + h64.size(), 1) * sizeof(HamJob)));
@@ -220,6 +235,16 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
     memcpy(st8, pjobs.data(), pjobs.size() * sizeof(PairJob));
     memcpy(st8 + pjobs.size() * sizeof(PairJob), h32.data(), h32.size() * sizeof(HamJob));
     memcpy(st8 + pjobs.size() * sizeof(PairJob) + h32.size() * sizeof(HamJob), h64.data(), h64.size() * sizeof(HamJob));
+    memcpy(st8 + jbytes, zero.data(), zero.size() * sizeof(uint32_t));
+    if (!zero.empty()) {
+        CR_TRY(slot_list.reserve(zero.size() * sizeof(uint32_t)));
+        CR_HIP(hipMemcpyAsync(slot_list.ptr, st8 + jbytes, zero.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              stream));
+        const uint32_t blocks = (uint32_t)std::min<size_t>(zero.size(), 1024);
+        hipLaunchKernelGGL(zero_slots_kernel, dim3(blocks), dim3(128), 0, stream, slab.as<unsigned long long>(),
+                           slot_list.as<uint32_t>(), (uint32_t)zero.size());
+        CR_HIP(hipGetLastError());
+    }
     if (!pjobs.empty())
         CR_HIP(hipMemcpyAsync(pair_jobs.ptr, st8, pjobs.size() * sizeof(PairJob), hipMemcpyHostToDevice, stream));
     if (!h32.empty() || !h64.empty())
@@ -278,6 +303,7 @@ void cr_server::loop() {
                     r->t_finish = now();
                     footprint -= r->footprint;
                     free_slots.push_back(r->slot);
+                    dirty_slots.push_back((uint32_t)r->slot);
                     st.cancelled++;
                     it = active.erase(it);
                 } else {
@@ -302,9 +328,7 @@ void cr_server::loop() {
                 footprint += r->footprint;
                 r->state = 1;
                 r->t_admit = now();
-                active.push_back(r->id);
-                // zero the slot before the request's first pass (stream order)
-                (void)fill_zero_u64(slab.as<unsigned long long>() + (size_t)r->slot * kSlotBins, kSlotBins, stream);
+                active.push_back(r->id);  // its slot is clean: zeroed at creation or when last freed
             }
             cv_done.notify_all();
             if (active.empty()) {
@@ -349,6 +373,7 @@ void cr_server::loop() {
             r->t_finish = now();
             footprint -= r->footprint;
             free_slots.push_back(r->slot);
+            dirty_slots.push_back((uint32_t)r->slot);
             st.completed++;
         }
         cv_done.notify_all();
@@ -371,16 +396,28 @@ cr_status cr_server_create(cr_context *ctx, const cr_server_config *cfg, cr_serv
     if (s->cfg.iteration_ms <= 0) s->cfg.iteration_ms = 50;
     if (pair_waves_per_cu(s->cfg.rows_per_lane, 66) == 0)
         return fail(CR_ERR_INVALID_ARGUMENT, "rows_per_lane must be 2, 4, 8 or 16");
-    CR_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    CR_HIP(hipEventCreate(&s->ev0));
-    CR_HIP(hipEventCreate(&s->evm));
-    CR_HIP(hipEventCreate(&s->ev1));
-    CR_TRY(s->slab.reserve((size_t)s->cfg.max_active * kSlotBins * sizeof(uint64_t)));
-    CR_TRY(s->counter.reserve(4 * sizeof(uint64_t)));
-    CR_TRY(s->readback.reserve((size_t)s->cfg.max_active * kSlotBins * sizeof(uint64_t)));
-    for (int i = (int)s->cfg.max_active - 1; i >= 0; --i) s->free_slots.push_back(i);
-    s->t0 = std::chrono::steady_clock::now();
+    // from here on a failure releases what was created (no worker yet)
     cr_server *raw = s.release();
+    cr_status st = [raw]() -> cr_status {
+        CR_HIP(hipStreamCreateWithFlags(&raw->stream, hipStreamNonBlocking));
+        CR_HIP(hipEventCreate(&raw->ev0));
+        CR_HIP(hipEventCreate(&raw->evm));
+        CR_HIP(hipEventCreate(&raw->ev1));
+        const size_t slots = (size_t)raw->cfg.max_active * kSlotBins;
+        CR_TRY(raw->slab.reserve(slots * sizeof(uint64_t)));
+        CR_TRY(fill_zero_u64(raw->slab.as<unsigned long long>(), slots, raw->stream));
+        CR_HIP(hipStreamSynchronize(raw->stream));
+        CR_TRY(raw->counter.reserve(4 * sizeof(uint64_t)));
+        CR_TRY(raw->readback.reserve(slots * sizeof(uint64_t)));
+        return CR_OK;
+    }();
+    if (st != CR_OK) {
+        const std::string msg = cr_last_error();
+        cr_server_destroy(raw);
+        return fail(st, "%s", msg.c_str());
+    }
+    for (int i = (int)raw->cfg.max_active - 1; i >= 0; --i) raw->free_slots.push_back(i);
+    raw->t0 = std::chrono::steady_clock::now();
     raw->worker = std::thread([raw] { raw->loop(); });
     *out = raw;
     return CR_OK;
@@ -492,6 +529,7 @@ cr_status cr_server_destroy(cr_server *s) {
     s->pair_jobs.release();
     s->ham_jobs.release();
     s->counter.release();
+    s->slot_list.release();
     s->staging.release();
     s->readback.release();
     if (s->ev0) (void)hipEventDestroy(s->ev0);
