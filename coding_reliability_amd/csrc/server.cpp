@@ -352,20 +352,27 @@ void cr_server::loop() {
                 }
             }
         }
-        // copy every finished histogram back, then synchronise once
+        // one copy of the slab range holding every finished histogram
+        int lo = 0, hi = -1;
+        if (!fin.empty()) {
+            lo = hi = fin[0]->slot;
+            for (Request *r : fin) {
+                lo = std::min(lo, r->slot);
+                hi = std::max(hi, r->slot);
+            }
+        }
         if (s == CR_OK && !fin.empty()) {
-            hipError_t e = hipSuccess;
-            for (size_t f = 0; f < fin.size() && e == hipSuccess; ++f)
-                e = hipMemcpyAsync(readback.as<uint64_t>() + f * kSlotBins,
-                                   slab.as<unsigned long long>() + (size_t)fin[f]->slot * kSlotBins,
-                                   fin[f]->nbins * sizeof(uint64_t), hipMemcpyDeviceToHost, stream);
+            hipError_t e = hipMemcpyAsync(readback.ptr, slab.as<unsigned long long>() + (size_t)lo * kSlotBins,
+                                          (size_t)(hi - lo + 1) * kSlotBins * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                          stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             if (e != hipSuccess) s = fail(CR_ERR_DEVICE, "result copy failed: %s", hipGetErrorString(e));
         }
-        for (size_t f = 0; f < fin.size(); ++f) {
-            Request *r = fin[f];
+        for (Request *r : fin) {
             std::vector<uint64_t> h(r->nbins, 0);
-            if (s == CR_OK) memcpy(h.data(), readback.as<uint64_t>() + f * kSlotBins, r->nbins * sizeof(uint64_t));
+            if (s == CR_OK)
+                memcpy(h.data(), readback.as<uint64_t>() + (size_t)(r->slot - lo) * kSlotBins,
+                       r->nbins * sizeof(uint64_t));
             std::lock_guard<std::mutex> lk(mu);
             r->result.swap(h);
             r->status = s;
