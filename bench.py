@@ -341,7 +341,7 @@ def run_served_small(ctx, njobs: int = 2048, seed: int = 11) -> dict:
     t0 = time.perf_counter()
     per = [ctx.an_pairs(A, 1 << k)[0] for A, k in jobs]
     t_per = time.perf_counter() - t0
-    srv = cra.Server(ctx, iteration_ms=2, max_active=256)
+    srv = cra.Server(ctx, iteration_ms=5, max_active=1024)
     t0 = time.perf_counter()
     ids = [srv.submit(kind=cra.JOB_AN_EXHAUSTIVE, k=k, A=A) for A, k in jobs]
     for rid in ids:
