@@ -90,7 +90,7 @@ def test_invalid_arguments():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("k", [8, 16, 26, 27, 32, 33, 36])
+@pytest.mark.parametrize("k", [8, 16, 26, 27, 32, 33, 36, 40])
 def test_device_enumeration_equals_macwilliams(k):
     ctx = cra.Context(0)
     try:
