@@ -16,6 +16,8 @@
 //     order, which is conflict free too; counters are flushed to the 64-bit
 //     device histogram with one atomic per bin per flush, before any 32-bit
 //     lane counter can wrap.
+#include <atomic>
+
 #include "cr_device.h"
 #include "cr_internal.h"
 
@@ -263,11 +265,17 @@ void pair_job_geometry(PairJob &job, uint32_t R, uint32_t Q) {
 }
 
 uint32_t pair_waves_per_cu(uint32_t R, uint32_t lds_rows) {
+    // one occupancy query per (R, rows): the answer is a property of the
+    // kernel and of gfx950's CU, the only target
+    static std::atomic<uint32_t> cache[17][67];
     const void *k = pair_kernel_for(R);
-    if (!k) return 0;
+    if (!k || lds_rows > 66) return 0;
+    const uint32_t hit = cache[R][lds_rows].load(std::memory_order_relaxed);
+    if (hit) return hit;
     int blocks = 0;
     const size_t lds = (size_t)lds_rows * kStride * sizeof(uint32_t);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, k, 64, lds) != hipSuccess) return 0;
+    cache[R][lds_rows].store((uint32_t)blocks, std::memory_order_relaxed);
     return (uint32_t)blocks;
 }
 
@@ -296,7 +304,7 @@ cr_status launch_pairs(const PairJob *d_jobs, uint32_t njobs, uint64_t total_ite
     }
     if ((uint64_t)waves > total_items) waves = (uint32_t)(total_items ? total_items : 1);
     const size_t lds = (size_t)cfg.lds_rows * kStride * sizeof(uint32_t);
-    CR_TRY(fill_zero_u64(counter, 1, stream));
+    if (cfg.zero_counter) CR_TRY(fill_zero_u64(counter, 1, stream));
     if (ev0) CR_HIP(hipEventRecord(ev0, stream));
     const PairJob *a_jobs = d_jobs;
     uint32_t a_njobs = njobs, a_rows = cfg.lds_rows;

@@ -74,22 +74,7 @@ static void auto_pair_tuning(uint64_t N, uint32_t resident_waves, uint32_t &R, u
     }
 }
 
-cr_status run_pair_jobs(cr_context *ctx, std::vector<PairJob> &jobs, const PairLaunchConfig &cfg_in,
-                        double *kernel_ms) {
-    PairLaunchConfig cfg = cfg_in;
-    uint64_t total = 0;
-    for (auto &j : jobs) {
-        j.item_begin = total;
-        total += j.items;
-    }
-    CR_TRY(ctx->jobs.reserve(jobs.size() * sizeof(PairJob)));
-    CR_TRY(ctx->counters.reserve(sizeof(unsigned long long)));
-    CR_HIP(hipMemcpyAsync(ctx->jobs.ptr, jobs.data(), jobs.size() * sizeof(PairJob), hipMemcpyHostToDevice,
-                          ctx->stream));
-    CR_TRY(launch_pairs(ctx->jobs.as<PairJob>(), (uint32_t)jobs.size(), total, ctx->counters.as<unsigned long long>(),
-                        cfg, ctx->cus, ctx->stream, kernel_ms ? ctx->ev0 : nullptr, kernel_ms ? ctx->ev1 : nullptr));
-    return CR_OK;
-}
+
 
 }  // namespace cr
 
@@ -203,16 +188,27 @@ cr_status cr_an_pairs_tuned(cr_context *ctx, uint64_t A, uint64_t num_words, uin
     job.nbins = need;
     pair_job_geometry(job, R, Q);
 
-    CR_TRY(ctx->hist.reserve(need * sizeof(uint64_t)));
+    // device: histogram bins then the work-queue head, cleared by one fill;
+    // host (pinned): the job descriptor at 0, the histogram from 256
+    constexpr size_t kHostResult = 256;
+    static_assert(sizeof(PairJob) <= kHostResult, "job staging");
+    CR_TRY(ctx->hist.reserve((need + 1) * sizeof(uint64_t)));
+    CR_TRY(ctx->jobs.reserve(sizeof(PairJob)));
+    CR_TRY(ctx->host.reserve(kHostResult + need * sizeof(uint64_t)));
     job.out = ctx->hist.as<unsigned long long>();
-    CR_TRY(fill_zero_u64(job.out, need, ctx->stream));
-    std::vector<PairJob> jobs{job};
-    CR_TRY(run_pair_jobs(ctx, jobs, cfg, kernel_ms));
-    CR_TRY(ctx->host.reserve(need * sizeof(uint64_t)));
-    CR_HIP(hipMemcpyAsync(ctx->host.ptr, job.out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long *queue = job.out + need;
+    job.item_begin = 0;
+    memcpy(ctx->host.ptr, &job, sizeof(job));
+    CR_TRY(fill_zero_u64(job.out, need + 1, ctx->stream));
+    CR_HIP(hipMemcpyAsync(ctx->jobs.ptr, ctx->host.ptr, sizeof(PairJob), hipMemcpyHostToDevice, ctx->stream));
+    cfg.zero_counter = false;
+    CR_TRY(launch_pairs(ctx->jobs.as<PairJob>(), 1, job.items, queue, cfg, ctx->cus, ctx->stream,
+                        kernel_ms ? ctx->ev0 : nullptr, kernel_ms ? ctx->ev1 : nullptr));
+    char *res = ctx->host.as<char>() + kHostResult;
+    CR_HIP(hipMemcpyAsync(res, job.out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     CR_HIP(hipStreamSynchronize(ctx->stream));
     memset(raw, 0, nbins * sizeof(uint64_t));
-    memcpy(raw, ctx->host.ptr, need * sizeof(uint64_t));
+    memcpy(raw, res, need * sizeof(uint64_t));
     if (kernel_ms) {
         float ms = 0;
         CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

@@ -99,8 +99,9 @@ uint32_t pair_mode(uint64_t A, uint64_t N);
 
 struct PairLaunchConfig {
     uint32_t R = 8;
-    uint32_t waves = 0;     // 0 = resident capacity of the device
-    uint32_t lds_rows = 0;  // histogram rows in LDS (max nbins + 1 dummy row)
+    uint32_t waves = 0;         // 0 = resident capacity of the device
+    uint32_t lds_rows = 0;      // histogram rows in LDS (max nbins + 1 dummy row)
+    bool zero_counter = true;   // false: the caller cleared the queue head already
 };
 
 // Launches zero-fill + persistent pair kernel on `stream` for the device job
