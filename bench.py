@@ -422,8 +422,8 @@ def main() -> int:
         return traffic_probe(args.A, args.k)
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is design-only, see DESIGN.md)"}))
+        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1 "
+                                   "(the pair space would shard by work items across GPUs; not built)"}))
         return 2
 
     import coding_reliability_amd as cra

@@ -24,7 +24,7 @@ extern "C" {
 typedef enum cr_status {
     CR_OK = 0,
     CR_ERR_INVALID_ARGUMENT = 1, /* bad size, width, or pointer */
-    CR_ERR_UNSUPPORTED = 2,      /* valid request outside what this build supports */
+    CR_ERR_UNSUPPORTED = 2,      /* valid request outside what this library supports */
     CR_ERR_NO_DEVICE = 3,        /* no gfx950 device at the given index */
     CR_ERR_OUT_OF_MEMORY = 4,    /* device allocation failed */
     CR_ERR_DEVICE = 5,           /* a HIP runtime call failed */
