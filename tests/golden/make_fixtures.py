@@ -13,6 +13,8 @@ Sources (paths relative to the reference root):
   distance_distribution/results/GridMs/*grid1d*.csv  deterministic grid1d runs
   distance_distribution/results/superAs/*.txt        super-A sweep summaries
   distance_distribution/legacy/print_codeword_matrix.out   k=5 full histograms
+  results/runtimes/4/ancoding_gpu_A61_n16_h6.csv, results/converge/geom/
+    ancoding_grid1d_gpu_m1001_A61_n16_h6.csv        CSV text (row and summary lines)
   distance_distribution/legacy/hamming_count_undetectable_errors.cpp
                                                      (40,33) and (52,45) weight tables
 
@@ -165,6 +167,29 @@ def legacy_hamming_weights():
     return res
 
 
+def recorded_csv_text():
+    """Row lines and summary lines of two recorded result files whose every
+    column is deterministic (counts, probabilities, binomial bases, and for the
+    grid run the error columns), for line-by-line comparison of the CSV
+    layout; the runtime lines are left out."""
+    files = {
+        "exhaustive_A61_n16": "results/runtimes/4/ancoding_gpu_A61_n16_h6.csv",
+        "grid1d_m1001_A61_n16": "results/converge/geom/ancoding_grid1d_gpu_m1001_A61_n16_h6.csv",
+    }
+    out = {}
+    for name, rel in files.items():
+        lines = open(os.path.join(DD, rel)).read().split("\n")
+        rows = []
+        for ln in lines:
+            if not ln.strip():
+                break
+            rows.append(ln)
+        summary = [ln for ln in lines if ln.startswith('"Total"') or ln.startswith('"MaxAbsErr"')
+                   or ln.startswith('"MaxRelErr"')]
+        out[name] = dict(src="distance_distribution/" + rel, rows=rows, summary=summary)
+    return out
+
+
 def main():
     sol = parse_solutions()
     runs, excluded = exhaustive_runs()
@@ -174,6 +199,7 @@ def main():
     json.dump(dict(runs=grid_runs()), open(os.path.join(OUT, "an_grid1d_runs.json"), "w"))
     json.dump(dict(files=supera()), open(os.path.join(OUT, "supera_runs.json"), "w"))
     json.dump(dict(matrices=codeword_matrices()), open(os.path.join(OUT, "codeword_matrices.json"), "w"), indent=1)
+    json.dump(recorded_csv_text(), open(os.path.join(OUT, "recorded_csv_text.json"), "w"), indent=1)
     json.dump(dict(tables=legacy_hamming_weights()), open(os.path.join(OUT, "legacy_hamming_weights.json"), "w"),
               indent=1)
     print(f"solutions: {len(sol)} vectors; exhaustive runs kept {len(runs)}, excluded {len(excluded)}")

@@ -91,3 +91,30 @@ def test_cli_an_mc_close_to_golden(tmp_path, solutions):
     assert abs(sum(got) - 4 ** 16) < 4 ** 16 * 1e-6 + 64
     for i in range(5, 20):
         assert abs(got[i] - sol[i]) <= 0.02 * sol[i], i
+
+
+def _text(path):
+    lines = open(path).read().split("\n")
+    rows = []
+    for ln in lines:
+        if not ln.strip():
+            break
+        rows.append(ln)
+    summary = [ln for ln in lines if ln.startswith(('"Total"', '"MaxAbsErr"', '"MaxRelErr"'))]
+    return rows, summary
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+@pytest.mark.parametrize("name,args,fname", [
+    ("exhaustive_A61_n16", ["-a", "61", "-n", "16", "-f"], "ancoding_gpu_A61_n16_h6.csv"),
+    ("grid1d_m1001_A61_n16", ["-a", "61", "-n", "16", "-m", "1001", "-g", "1", "-f"],
+     "ancoding_grid1d_gpu_m1001_A61_n16_h6.csv"),
+])
+def test_cli_csv_text_matches_recorded_file(tmp_path, name, args, fname):
+    # every column of every row, and the total / error summary lines, as text
+    rec = golden("recorded_csv_text.json")[name]
+    run_cli(tmp_path, *args)
+    rows, summary = _text(tmp_path / fname)
+    assert rows == rec["rows"], rec["src"]
+    assert summary == rec["summary"], rec["src"]
