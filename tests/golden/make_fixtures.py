@@ -176,6 +176,10 @@ def recorded_csv_text():
         "exhaustive_A61_n16": "results/runtimes/4/ancoding_gpu_A61_n16_h6.csv",
         "grid1d_m1001_A61_n16": "results/converge/geom/ancoding_grid1d_gpu_m1001_A61_n16_h6.csv",
     }
+    for A in (1, 3, 17, 63, 99, 127):
+        for n in (8, 12, 16):
+            for f in glob.glob(os.path.join(DD, "results", "As", f"ancoding_gpu_A{A}_n{n}_h*.csv")):
+                files[f"As_A{A}_n{n}"] = os.path.relpath(f, DD)
     out = {}
     for name, rel in files.items():
         lines = open(os.path.join(DD, rel)).read().split("\n")

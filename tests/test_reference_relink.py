@@ -118,3 +118,24 @@ def test_cli_csv_text_matches_recorded_file(tmp_path, name, args, fname):
     rows, summary = _text(tmp_path / fname)
     assert rows == rec["rows"], rec["src"]
     assert summary == rec["summary"], rec["src"]
+
+
+def _as_cases():
+    import re
+    cases = []
+    for name, rec in sorted(golden("recorded_csv_text.json").items()):
+        m = re.match(r"As_A(\d+)_n(\d+)$", name)
+        if m:
+            cases.append((name, m.group(1), m.group(2), os.path.basename(rec["src"])))
+    return cases
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+@pytest.mark.parametrize("name,A,n,fname", _as_cases())
+def test_cli_csv_text_matches_recorded_as_files(tmp_path, name, A, n, fname):
+    rec = golden("recorded_csv_text.json")[name]
+    run_cli(tmp_path, "-a", A, "-n", n, "-f")
+    rows, summary = _text(tmp_path / fname)
+    assert rows == rec["rows"], rec["src"]
+    assert summary == rec["summary"], rec["src"]
