@@ -80,3 +80,34 @@ def test_batch_invalid(ctx):
         ctx.an_batch([(0, 16)])
     with pytest.raises(cra.CRError):
         ctx.an_batch([((1 << 63) + 1, 4)])
+
+
+def test_repeated_create_destroy_leaks_nothing():
+    # contexts, batch plans and servers created and destroyed many times:
+    # device memory in use returns to where it started
+    import torch
+
+    import coding_reliability_amd as cra
+
+    def used():
+        torch.cuda.synchronize()
+        free, total = torch.cuda.mem_get_info(0)
+        return total - free
+
+    codes = [(61, 1 << 10), (58659, 1 << 12)]
+    c0 = cra.Context(0)
+    ref = c0.an_batch(codes)[0]
+    c0.close()
+    base = used()
+    for _ in range(20):
+        c = cra.Context(0)
+        assert c.an_batch(codes)[0] == ref
+        p = c.batch_plan(codes)
+        assert p.run()[0] == ref
+        p.close()
+        with cra.Server(c, max_active=32) as srv:
+            rid = srv.submit(kind=cra.JOB_AN_EXHAUSTIVE, k=10, A=61)
+            srv.wait(rid, timeout_ms=30000)
+            assert srv.result(rid)[0] == ref[0][: len(srv.result(rid)[0])]
+        c.close()
+    assert used() - base < (64 << 20)
