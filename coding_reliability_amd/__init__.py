@@ -124,6 +124,12 @@ class Context:
         return dict(name=info.name.decode(), arch=info.arch.decode(), compute_units=info.compute_units,
                     max_clock_mhz=info.max_clock_mhz, hbm_bytes=info.hbm_bytes)
 
+    def device_memory(self) -> tuple[int, int]:
+        """(free, total) device memory in bytes."""
+        f, t = C.c_uint64(0), C.c_uint64(0)
+        check(lib().cr_device_memory(self._h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
     def an_pairs(self, A: int, num_words: int, nbins: int | None = None, rows_per_lane: int = 0,
                  chunk_blocks: int = 0, waves: int = 0) -> tuple[list[int], float]:
         """Raw unordered-pair histogram of codewords A*i, i < num_words, and the

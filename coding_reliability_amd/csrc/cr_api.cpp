@@ -149,6 +149,17 @@ cr_status cr_context_destroy(cr_context *ctx) {
     return CR_OK;
 }
 
+cr_status cr_device_memory(cr_context *ctx, uint64_t *free_bytes, uint64_t *total_bytes) {
+    clear_error();
+    if (!ctx || !free_bytes || !total_bytes) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    CR_HIP(hipSetDevice(ctx->device));
+    size_t f = 0, t = 0;
+    CR_HIP(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+    return CR_OK;
+}
+
 cr_status cr_device_query(cr_context *ctx, cr_device_info *out) {
     clear_error();
     if (!ctx || !out) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");

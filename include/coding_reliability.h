@@ -60,6 +60,11 @@ typedef struct cr_device_info {
 } cr_device_info;
 cr_status cr_device_query(cr_context *ctx, cr_device_info *out);
 
+/* Free and total device memory (bytes) of the context's device, as the HIP
+ * runtime reports them; for sizing server budgets and leak checks.  No
+ * counterpart in the reference, which allocated per call. */
+cr_status cr_device_memory(cr_context *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
+
 /* --- exhaustive AN ---------------------------------------------------------
  * raw[b] = #{ 0 <= i < j < num_words : popcount(A*i XOR A*j) == b } for
  * b < nbins, computed exactly on the GPU (uint64 counts).

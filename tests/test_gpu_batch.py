@@ -85,13 +85,12 @@ def test_batch_invalid(ctx):
 def test_repeated_create_destroy_leaks_nothing():
     # contexts, batch plans and servers created and destroyed many times:
     # device memory in use returns to where it started
-    import torch
-
     import coding_reliability_amd as cra
 
+    probe = cra.Context(0)
+
     def used():
-        torch.cuda.synchronize()
-        free, total = torch.cuda.mem_get_info(0)
+        free, total = probe.device_memory()
         return total - free
 
     codes = [(61, 1 << 10), (58659, 1 << 12)]
@@ -111,3 +110,4 @@ def test_repeated_create_destroy_leaks_nothing():
             assert srv.result(rid)[0] == ref[0][: len(srv.result(rid)[0])]
         c.close()
     assert used() - base < (64 << 20)
+    probe.close()
