@@ -66,11 +66,18 @@ __device__ __forceinline__ uint32_t weight_tab(uint64_t d, uint64_t dmask, const
     return __popc(syn) + __popc(lo & (uint32_t)dmask);
 }
 
-// Syndrome of a 32-bit word: bit t = parity(x & m_t), t < 6.
+// Syndrome of a data word: bit t = parity(x & m_t) (the seventh mask is zero
+// for k <= 32).
 __device__ __forceinline__ uint32_t syndrome32(uint32_t x, const uint64_t (&m)[7]) {
     uint32_t syn = 0;
 #pragma unroll
     for (int t = 0; t < 6; ++t) syn |= (__popc(x & (uint32_t)m[t]) & 1u) << t;
+    return syn;
+}
+__device__ __forceinline__ uint32_t syndrome64(uint64_t x, const uint64_t (&m)[7]) {
+    uint32_t syn = 0;
+#pragma unroll
+    for (int t = 0; t < 7; ++t) syn |= (uint32_t)(__popcll(x & m[t]) & 1) << t;
     return syn;
 }
 
@@ -162,7 +169,7 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
                 mc_item<W64, false>(lane_col, tab, job, base, spl, count, dmask, dummy);
             else
                 mc_item<W64, true>(lane_col, tab, job, base, spl, count, dmask, dummy);
-        } else if (!W64 && job.source == kHamRange && base + 64ull * spl <= count && spl >= kFastUnroll &&
+        } else if (job.source == kHamRange && base + 64ull * spl <= count && spl >= kFastUnroll &&
                    (spl & (spl - 1)) == 0 && ((job.first + base) & (64ull * spl - 1)) == 0 &&
                    64ull * spl - 1 <= dmask) {
             // full item of consecutive words (exhaustive) whose first word is a
@@ -177,10 +184,18 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
                 }
                 fast_job = jb;
             }
-            const uint32_t d0 = (uint32_t)((job.first + base) & dmask);
+            const uint64_t d0 = (job.first + base) & dmask;
             for (uint32_t qh = 0; qh < spl / kFastUnroll; ++qh) {
-                const uint32_t dh = uniform32(d0 | (qh << 10));
-                const uint32_t sh = uniform32(syndrome32(dh, m)), ph = uniform32(__popc(dh));
+                uint32_t sh, ph;
+                if (W64) {
+                    const uint64_t dh = uniform64(d0 | ((uint64_t)qh << 10));
+                    sh = uniform32(syndrome64(dh, m));
+                    ph = uniform32((uint32_t)__popcll(dh));
+                } else {
+                    const uint32_t dh = uniform32((uint32_t)d0 | (qh << 10));
+                    sh = uniform32(syndrome32(dh, m));
+                    ph = uniform32(__popc(dh));
+                }
 #pragma unroll
                 for (uint32_t ql = 0; ql < kFastUnroll; ++ql) {
                     const uint32_t w = __popc(fsyn[ql] ^ sh) + ph;
