@@ -146,3 +146,16 @@ def test_cli_csv_text_matches_recorded_as_files(tmp_path, name, A, n, fname):
     assert summary == [f'"Total",{float(sum(counts)):.6g}']
     if rec["summary"]:
         assert rec["summary"][0] in (f'"Total",{float(sum(counts)):.6g}', f'"Total",{float(sum(counts[:-1])):.6g}')
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+def test_cli_grid_k40_csv_text_matches_recorded_file(tmp_path):
+    # 2^40 words x 101 reference samples (1.2e14 pairs): counts above 2^64
+    # printed with 12 digits; the recorded error columns are uninitialised
+    # memory, so the first four columns of each row and the total are compared
+    rec = golden("recorded_csv_text.json")["grid1d_m101_A61_n40"]
+    run_cli(tmp_path, "-a", "61", "-n", "40", "-m", "101", "-g", "1", "-f")
+    rows, summary = _text(tmp_path / "ancoding_grid1d_gpu_m101_A61_n40_h6.csv")
+    assert [",".join(r.split(",")[:4]) for r in rows] == rec["rows"], rec["src"]
+    assert [x for x in summary if x.startswith('"Total"')] == rec["summary"], rec["src"]
