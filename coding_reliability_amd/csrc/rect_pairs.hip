@@ -116,7 +116,7 @@ template <int R, bool W2>
 __global__ __launch_bounds__(64) void rect_kernel(uint64_t A, uint64_t row0, uint64_t nrows,
                                                   const uint64_t *__restrict__ rows,
                                                   const uint64_t *__restrict__ cols, uint64_t ncols, uint32_t CW,
-                                                  uint64_t CC, uint64_t total_items,
+                                                  uint64_t CC, uint64_t G, uint64_t RBlocks, uint64_t total_items,
                                                   unsigned long long *__restrict__ counter,
                                                   unsigned long long *__restrict__ out, uint32_t nbins,
                                                   uint32_t first_bin, uint32_t lds_rows, uint32_t mask32) {
@@ -133,32 +133,36 @@ __global__ __launch_bounds__(64) void rect_kernel(uint64_t A, uint64_t row0, uin
     for (;;) {
         const uint64_t t = next_item(counter);
         if (t >= total_items) break;
-        const uint64_t rb = uniform64(t / CC), cc = uniform64(t - (t / CC) * CC);
+        // item = (group of G row blocks, column chunk)
+        const uint64_t rbg = uniform64(t / CC), cc = uniform64(t - (t / CC) * CC);
         const uint64_t j0 = cc * CW;
         const uint64_t j1 = j0 + CW < ncols ? j0 + CW : ncols;
         const uint32_t n = uniform32((uint32_t)(j1 - j0));
-        if ((pending + n) * R >= 0xFFFFFFFFull) {
-            flush_rows(lds, first_bin, nbins, out);
-            pending = 0;
-        }
-        pending += n;
-        dirty = true;
+        const uint64_t rb_end = rbg * G + G < RBlocks ? rbg * G + G : RBlocks;
+        for (uint64_t rb = rbg * G; rb < rb_end; ++rb) {
+            if ((pending + n) * R >= 0xFFFFFFFFull) {
+                flush_rows(lds, first_bin, nbins, out);
+                pending = 0;
+            }
+            pending += n;
+            dirty = true;
 
-        uint64_t w[R];
-        bool ok[R];
-        bool full = true;
+            uint64_t w[R];
+            bool ok[R];
+            bool full = true;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const uint64_t i = rb * RB + (uint64_t)r * 64 + lane;
-            ok[r] = i < nrows;
-            full = full && ((rb * RB + (uint64_t)r * 64 + 63) < nrows);
-            const uint64_t ii = ok[r] ? i : 0;
-            w[r] = (rows ? rows[ii] : A * (row0 + ii)) & wmask;
+            for (int r = 0; r < R; ++r) {
+                const uint64_t i = rb * RB + (uint64_t)r * 64 + lane;
+                ok[r] = i < nrows;
+                full = full && ((rb * RB + (uint64_t)r * 64 + 63) < nrows);
+                const uint64_t ii = ok[r] ? i : 0;
+                w[r] = (rows ? rows[ii] : A * (row0 + ii)) & wmask;
+            }
+            if (full)
+                rect_cols<R, W2>(lane_col, w, cols, j0, n);
+            else
+                rect_cols_masked<R>(lane_col, w, ok, cols, j0, n, dummy);
         }
-        if (full)
-            rect_cols<R, W2>(lane_col, w, cols, j0, n);
-        else
-            rect_cols_masked<R>(lane_col, w, ok, cols, j0, n, dummy);
     }
     if (dirty) flush_rows(lds, first_bin, nbins, out);
 }
@@ -190,16 +194,21 @@ cr_status launch_rect(const RectLaunch &L, hipStream_t stream, hipEvent_t ev0, h
     if (CW > (1u << 22)) CW = 1u << 22;
     if (CW < 1) CW = 1;
     CC = (L.ncols + CW - 1) / CW;
-    const uint64_t items = RBlocks * CC;
+    // many more row blocks than waves (every word against a short sample
+    // list): G row blocks per item keep the items near the target count, so
+    // the work-queue counter (one atomic per item) stays off the critical path
+    const uint64_t G = RBlocks > target ? RBlocks / target : 1;
+    const uint64_t items = (RBlocks + G - 1) / G * CC;
     uint32_t waves = (uint32_t)std::min<uint64_t>(waves_cap, items);
     CR_TRY(fill_zero_u64(L.counter, 1, stream));
     if (ev0) CR_HIP(hipEventRecord(ev0, stream));
-    uint64_t a_A = L.A, a_row0 = L.row0, a_nrows = L.nrows, a_ncols = L.ncols, a_CC = CC, a_items = items;
+    uint64_t a_A = L.A, a_row0 = L.row0, a_nrows = L.nrows, a_ncols = L.ncols, a_CC = CC, a_items = items, a_G = G,
+             a_RB = RBlocks;
     const uint64_t *a_rows = L.rows, *a_cols = L.cols;
     uint32_t a_CW = (uint32_t)CW, a_nbins = L.nbins, a_first = L.first_bin, a_lr = L.lds_rows, a_m32 = L.mask32;
     unsigned long long *a_counter = L.counter, *a_out = L.out;
-    void *args[] = {&a_A, &a_row0, &a_nrows, (void *)&a_rows, (void *)&a_cols, &a_ncols, &a_CW, &a_CC,
-                    &a_items, &a_counter, &a_out, &a_nbins, &a_first, &a_lr, &a_m32};
+    void *args[] = {&a_A, &a_row0, &a_nrows, (void *)&a_rows, (void *)&a_cols, &a_ncols, &a_CW, &a_CC, &a_G,
+                    &a_RB, &a_items, &a_counter, &a_out, &a_nbins, &a_first, &a_lr, &a_m32};
     CR_HIP(hipLaunchKernel(k, dim3(waves), dim3(64), args, lds, stream));
     if (ev1) CR_HIP(hipEventRecord(ev1, stream));
     return CR_OK;
