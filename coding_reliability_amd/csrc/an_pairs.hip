@@ -158,10 +158,7 @@ __global__ __launch_bounds__(64) void pair_kernel(const PairJob *__restrict__ jo
     uint32_t cur = 0xFFFFFFFFu;  // job whose counts sit in LDS
     uint64_t pending = 0;        // columns accumulated since the last flush
 
-    for (;;) {
-        const uint64_t t = next_item(counter);
-        if (t >= total_items) break;
-
+    for (uint64_t t = first_item(); t < total_items; t = following_item(counter, t)) {
         // job lookup: current job first, else binary search over item_begin
         uint32_t jb;
         if (cur != 0xFFFFFFFFu && t >= jobs[cur].item_begin && t < jobs[cur].item_begin + jobs[cur].items) {
@@ -304,7 +301,17 @@ cr_status launch_pairs(const PairJob *d_jobs, uint32_t njobs, uint64_t total_ite
     }
     if ((uint64_t)waves > total_items) waves = (uint32_t)(total_items ? total_items : 1);
     const size_t lds = (size_t)cfg.lds_rows * kStride * sizeof(uint32_t);
-    if (cfg.zero_counter) CR_TRY(fill_zero_u64(counter, 1, stream));
+    if (cfg.allow_static && total_items > 0) {
+        // at most one wave per CU less than the resident capacity, each
+        // taking the same number of items
+        counter = nullptr;
+        const uint32_t per_cu = pair_waves_per_cu(cfg.R, cfg.lds_rows);
+        const uint64_t cap = (uint64_t)(per_cu > 1 ? per_cu - 1 : 1) * cus;
+        if (waves > cap) waves = (uint32_t)cap;
+        const uint64_t per = (total_items + waves - 1) / waves;
+        waves = (uint32_t)((total_items + per - 1) / per);
+    }
+    if (cfg.zero_counter && counter) CR_TRY(fill_zero_u64(counter, 1, stream));
     if (ev0) CR_HIP(hipEventRecord(ev0, stream));
     const PairJob *a_jobs = d_jobs;
     uint32_t a_njobs = njobs, a_rows = cfg.lds_rows;

@@ -61,9 +61,7 @@ __global__ __launch_bounds__(64) void cand_kernel(const uint64_t *__restrict__ a
                                                   unsigned long long *__restrict__ queue,
                                                   unsigned int *__restrict__ gbest) {
     const uint32_t lane = threadIdx.x;
-    for (;;) {
-        const uint64_t t = next_item(queue);
-        if (t >= total_items) break;
+    for (uint64_t t = first_item(); t < total_items; t = following_item(queue, t)) {
         const uint64_t g = uniform64(t / nchunks);
         const uint32_t ci = uniform32((uint32_t)(t - g * nchunks));
         const uint64_t idx = g * 64 + lane;

@@ -60,6 +60,9 @@ void HostBuffer::release() {
 // Tuning for one pair job when the caller leaves a knob at 0: enough items
 // (about 16 per resident wave) to balance the persistent waves, rows per lane
 // large enough to amortise the per-item set-up.
+// Single-code launches below this many pairs run with a static assignment.
+constexpr long double kStaticPairs = 1e9;
+
 static void auto_pair_tuning(uint64_t N, uint32_t resident_waves, uint32_t &R, uint32_t &Q) {
     if (R == 0) R = N >= (1ull << 16) ? 8 : (N >= (1ull << 11) ? 4 : 2);
     if (Q == 0) {
@@ -191,6 +194,11 @@ cr_status cr_an_pairs_tuned(cr_context *ctx, uint64_t A, uint64_t num_words, uin
     auto_pair_tuning(num_words, per_cu * (uint32_t)ctx->cus, R, Q);
     cfg.R = R;
     cfg.waves = tuning ? tuning->waves : 0;
+    // below about 100 us of work (1e9 pairs at 9.6e12 pairs/s) the queue's
+    // serialized atomics, one per item, cost more than a static assignment's
+    // imbalance
+    const long double pairs = (long double)num_words * (long double)(num_words - 1) / 2;
+    cfg.allow_static = cfg.waves == 0 && pairs < kStaticPairs;
 
     PairJob job{};
     job.A = A;

@@ -59,6 +59,17 @@ __device__ __forceinline__ uint64_t next_item(unsigned long long *counter) {
     return uniform64(t);
 }
 
+// Item order of the persistent kernels (every launch has at most as many
+// waves W = gridDim.x as items): wave b's first item is item b, taken without
+// an atomic; its later items come from the work queue at `counter`, whose
+// k-th step hands out item W + k, or, with counter == nullptr (a static
+// assignment), are b + W, b + 2W, ...  Each queue step is a device-scope
+// atomic on one address, serialized at about 12.5 ns on MI355X.
+__device__ __forceinline__ uint64_t first_item() { return blockIdx.x; }
+__device__ __forceinline__ uint64_t following_item(unsigned long long *counter, uint64_t t) {
+    return counter ? (uint64_t)gridDim.x + next_item(counter) : t + gridDim.x;
+}
+
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of the 4x32
 // Philox S-box, multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key increments
 // 0x9E3779B9 / 0xBB67AE85.  c is the counter on input, the output on return.

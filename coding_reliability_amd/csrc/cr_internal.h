@@ -102,6 +102,7 @@ struct PairLaunchConfig {
     uint32_t waves = 0;         // 0 = resident capacity of the device
     uint32_t lds_rows = 0;      // histogram rows in LDS (max nbins + 1 dummy row)
     bool zero_counter = true;   // false: the caller cleared the queue head already
+    bool allow_static = false;  // a short launch: static assignment of items to waves, no queue
 };
 
 // Launches zero-fill + persistent pair kernel on `stream` for the device job
@@ -143,11 +144,16 @@ struct HamJob {
     unsigned long long *out;   // device weight histogram (nbins entries)
     uint64_t masks[7];         // parity masks truncated to k bits, zero when unused
     uint32_t source, k, stream, nbins;
-    uint32_t lane_samples;     // samples per lane per work item (power of two in [4, kHamMaxLaneSamples])
-    uint32_t pad;
+    uint32_t lane_samples;     // samples per lane per work item (power of two in [16, kHamMaxLaneSamples])
+    uint32_t out_copies;       // copies of the histogram at out (power of two; 0 = 1), ham_copy_stride apart
 };
 static_assert(sizeof(HamJob) == 152, "HamJob layout");
 constexpr uint32_t kHamMaxLaneSamples = 4096;
+// Histogram copies of the standalone Hamming entry point: the flush atomics of
+// its waves land on 32 sets of cache lines.
+constexpr uint32_t kHamOutCopies = 32;
+// Entries between two copies of an nbins histogram: whole 128-byte lines.
+__host__ __device__ inline uint64_t ham_copy_stride(uint32_t nbins) { return ((uint64_t)nbins + 15) & ~15ull; }
 
 // Parity masks of the (N,k) extended Hamming codes, truncated to k bits;
 // returns the number of non-overall parity bits.
@@ -156,9 +162,12 @@ uint32_t hamming_masks(uint32_t k, uint64_t masks[7]);
 uint64_t hamming_items(uint64_t count, uint32_t lane_samples);
 // Waves per CU of the Hamming kernel (the larger of the 32- and 64-bit forms).
 uint32_t hamming_waves_per_cu(uint32_t lds_rows);
+// counter: the work-queue head (zeroed here).  allow_static: a short launch
+// of equal items, run with a static assignment of items to waves instead of
+// the queue.
 cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_items, bool w64,
-                         unsigned long long *counter, uint32_t lds_rows, int cus, hipStream_t stream, hipEvent_t ev0,
-                         hipEvent_t ev1);
+                         unsigned long long *counter, bool allow_static, uint32_t lds_rows, int cus,
+                         hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1);
 cr_status launch_an_mc(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, unsigned long long *counter,
                        unsigned long long *out, uint32_t nbins, int cus, hipStream_t stream, hipEvent_t ev0,
                        hipEvent_t ev1);

@@ -12,6 +12,7 @@
 // AN Monte Carlo: every word A*i against m Philox samples A*u_ij
 // (replaces dancoding_mc_shared, an_coding_mc.cu:94-158).
 #include <algorithm>
+#include <type_traits>
 
 #include "cr_device.h"
 #include "cr_internal.h"
@@ -66,6 +67,26 @@ __device__ __forceinline__ uint32_t weight_tab(uint64_t d, uint64_t dmask, const
     return __popc(syn) + __popc(lo & (uint32_t)dmask);
 }
 
+// Weights of U words at once: the byte-table reads of all U words are issued
+// before any of them is combined, so they are in flight together.
+template <bool W64, int U>
+__device__ __forceinline__ void weights_tab(const uint64_t (&d)[U], uint64_t dmask, const uint8_t *tab,
+                                            uint32_t (&w)[U]) {
+    constexpr int B = W64 ? 8 : 4;
+    uint32_t r[U][B];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int b = 0; b < B; ++b) r[u][b] = tab[b * 256 + (uint32_t)((d[u] >> (8 * b)) & 255)];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        uint32_t syn = r[u][0];
+#pragma unroll
+        for (int b = 1; b < B; ++b) syn ^= r[u][b];
+        w[u] = __popc(syn) + (W64 ? (uint32_t)__popcll(d[u] & dmask) : (uint32_t)__popc((uint32_t)d[u] & (uint32_t)dmask));
+    }
+}
+
 // Syndrome of a data word: bit t = parity(x & m_t) (the seventh mask is zero
 // for k <= 32).
 __device__ __forceinline__ uint32_t syndrome32(uint32_t x, const uint64_t (&m)[7]) {
@@ -102,21 +123,72 @@ __device__ __forceinline__ void mc_item(uint32_t *lane_col, const uint8_t *tab, 
         const uint64_t blk = (job.first + s) / per;
         uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), job.stream, kMcTag};
         philox4x32_10(c, k0, k1);
+        // the block's table reads are all issued before its increments
         if (per == 4) {
+            const uint64_t x[4] = {c[0], c[1], c[2], c[3]};
+            uint32_t w[4];
+            weights_tab<W64, 4>(x, dmask, tab, w);
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
-                const uint32_t w = weight_tab<W64>((uint64_t)c[u], dmask, tab);
-                lds_inc(lane_col + ((!CHECK || s + u < count) ? w : dummy) * kStride);
-            }
+            for (uint32_t u = 0; u < 4; ++u) lds_inc(lane_col + ((!CHECK || s + u < count) ? w[u] : dummy) * kStride);
         } else {
+            const uint64_t x[2] = {(uint64_t)c[0] | ((uint64_t)c[1] << 32), (uint64_t)c[2] | ((uint64_t)c[3] << 32)};
+            uint32_t w[2];
+            weights_tab<W64, 2>(x, dmask, tab, w);
 #pragma unroll
-            for (uint32_t u = 0; u < 2; ++u) {
-                const uint64_t d = (uint64_t)c[2 * u] | ((uint64_t)c[2 * u + 1] << 32);
-                const uint32_t w = weight_tab<W64>(d, dmask, tab);
-                lds_inc(lane_col + ((!CHECK || s + u < count) ? w : dummy) * kStride);
-            }
+            for (uint32_t u = 0; u < 2; ++u) lds_inc(lane_col + ((!CHECK || s + u < count) ? w[u] : dummy) * kStride);
         }
     }
+}
+
+// Range, stride and list items off the exhaustive fast path.  Lane l takes
+// the words d_q = first + (base + 64q + l) * stride for q < spl (stride 1 for
+// a range), so d advances by 64*stride per step (modulo 2^64, or 2^32 for
+// k <= 32: only the low k bits reach the weight), or list[base + 64q + l].
+// kSeqUnroll steps per iteration, all table reads (and list loads) of the
+// batch issued before its increments, so the reads overlap instead of each
+// waiting behind the previous increment.  spl is a multiple of kSeqUnroll.
+// CHECK: steps past the job's count go to the dummy row.
+constexpr uint32_t kSeqUnroll = 8;
+
+template <bool W64, bool CHECK, bool LIST>
+__device__ __forceinline__ void seq_item(uint32_t *lane_col, const uint8_t *tab, const HamJob &job, uint64_t base,
+                                         uint32_t spl, uint64_t count, uint64_t dmask, uint32_t dummy) {
+    using word_t = typename std::conditional<W64, uint64_t, uint32_t>::type;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t rem = count - base;
+    const uint32_t left = rem < 64ull * spl ? (uint32_t)rem : 64u * spl;  // valid samples in the item
+    const uint64_t stride = job.source == kHamRange ? 1 : job.stride;
+    word_t d = (word_t)(job.first + (base + lane) * stride);
+    const word_t step = (word_t)(64 * stride);
+    const uint64_t *list = LIST ? job.list + base + lane : nullptr;
+    for (uint32_t q = 0; q < spl; q += kSeqUnroll) {
+        uint64_t x[kSeqUnroll];
+        uint32_t w[kSeqUnroll];
+        if (LIST) {
+#pragma unroll
+            for (uint32_t u = 0; u < kSeqUnroll; ++u)
+                x[u] = (!CHECK || (q + u) * 64 + lane < left) ? list[(q + u) * 64] : 0;
+        } else {
+#pragma unroll
+            for (uint32_t u = 0; u < kSeqUnroll; ++u) {
+                x[u] = (uint64_t)d;
+                d += step;
+            }
+        }
+        weights_tab<W64, kSeqUnroll>(x, dmask, tab, w);
+#pragma unroll
+        for (uint32_t u = 0; u < kSeqUnroll; ++u)
+            lds_inc(lane_col + ((!CHECK || (q + u) * 64 + lane < left) ? w[u] : dummy) * kStride);
+    }
+}
+
+// A job's histogram is out_copies copies (a power of two; 0 means 1) of
+// ham_copy_stride(nbins) entries each; wave b flushes into copy b mod
+// out_copies, which spreads the flush atomics of many waves over several cache
+// lines instead of queueing them on the few lines of one histogram.
+__device__ __forceinline__ unsigned long long *copy_out(const HamJob &job) {
+    const uint32_t c = job.out_copies > 1 ? (blockIdx.x & (job.out_copies - 1)) : 0;
+    return job.out + (uint64_t)c * ham_copy_stride(job.nbins);
 }
 
 __device__ __forceinline__ uint32_t find_job(const HamJob *jobs, uint32_t njobs, uint64_t t, uint32_t cur) {
@@ -144,14 +216,12 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
     uint32_t fast_job = 0xFFFFFFFFu;  // job whose per-lane constants are loaded
     uint32_t fsyn[kFastUnroll], fbase[kFastUnroll];
 
-    for (;;) {
-        const uint64_t t = next_item(counter);
-        if (t >= total_items) break;
+    for (uint64_t t = first_item(); t < total_items; t = following_item(counter, t)) {
         const uint32_t jb = find_job(jobs, njobs, t, cur);
         const HamJob &job = jobs[jb];
         const uint32_t spl = job.lane_samples;
         if (jb != cur || pending + spl >= 0xFFFFFFFFull) {
-            if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, jobs[cur].out);
+            if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, copy_out(jobs[cur]));
             if (jb != cur) build_tables<W64>(tab, job.masks);
             cur = jb;
             pending = 0;
@@ -203,22 +273,20 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
                 }
             }
         } else {
-            for (uint32_t q = 0; q < spl; ++q) {
-                const uint64_t s = base + (uint64_t)q * 64 + lane;
-                const bool ok = s < count;
-                uint64_t d;
-                if (job.source == kHamRange)
-                    d = job.first + s;
-                else if (job.source == kHamStride)
-                    d = job.first + s * job.stride;
+            const bool full = base + 64ull * spl <= count;
+            if (job.source == kHamList) {
+                if (full)
+                    seq_item<W64, false, true>(lane_col, tab, job, base, spl, count, dmask, dummy);
                 else
-                    d = ok ? job.list[s] : 0;
-                const uint32_t w = weight_tab<W64>(d, dmask, tab);
-                lds_inc(lane_col + (ok ? w : dummy) * kStride);
+                    seq_item<W64, true, true>(lane_col, tab, job, base, spl, count, dmask, dummy);
+            } else if (full) {
+                seq_item<W64, false, false>(lane_col, tab, job, base, spl, count, dmask, dummy);
+            } else {
+                seq_item<W64, true, false>(lane_col, tab, job, base, spl, count, dmask, dummy);
             }
         }
     }
-    if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, jobs[cur].out);
+    if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, copy_out(jobs[cur]));
 }
 
 // AN Monte Carlo (n <= 32): item = (row block of 64*RM words, sample chunk
@@ -271,9 +339,7 @@ __global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint6
     const bool w1 = ((unsigned __int128)A * (nwords - 1)) >> 32 == 0;
     uint64_t pending = 0;
     bool dirty = false;
-    for (;;) {
-        const uint64_t t = next_item(counter);
-        if (t >= total_items) break;
+    for (uint64_t t = first_item(); t < total_items; t = following_item(counter, t)) {
         const uint64_t rb = uniform64(t / SC), sc = uniform64(t - (t / SC) * SC);
         const uint64_t j0 = sc * CJ;
         const uint64_t j1 = j0 + CJ < m ? j0 + CJ : m;
@@ -302,8 +368,8 @@ size_t ham_lds_bytes(bool w64, uint32_t lds_rows) {
 }  // namespace
 
 cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_items, bool w64,
-                         unsigned long long *counter, uint32_t lds_rows, int cus, hipStream_t stream, hipEvent_t ev0,
-                         hipEvent_t ev1) {
+                         unsigned long long *counter, bool allow_static, uint32_t lds_rows, int cus,
+                         hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1) {
     const void *k = w64 ? reinterpret_cast<const void *>(&ham_kernel<true>)
                         : reinterpret_cast<const void *>(&ham_kernel<false>);
     if (lds_rows < 2 || lds_rows > 131) return fail(CR_ERR_INVALID_ARGUMENT, "bad histogram rows %u", lds_rows);
@@ -314,7 +380,16 @@ cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_it
     uint64_t waves = (uint64_t)per_cu * cus;
     if (waves > total_items) waves = total_items;
     if (waves == 0) waves = 1;
-    CR_TRY(fill_zero_u64(counter, 1, stream));
+    if (allow_static && total_items > 0) {
+        // static assignment over at most one wave per CU less than the
+        // resident capacity, each wave taking the same number of items
+        counter = nullptr;
+        const uint64_t cap = (uint64_t)(per_cu > 1 ? per_cu - 1 : 1) * cus;
+        if (waves > cap) waves = cap;
+        const uint64_t per = (total_items + waves - 1) / waves;
+        waves = (total_items + per - 1) / per;
+    }
+    if (counter) CR_TRY(fill_zero_u64(counter, 1, stream));
     if (ev0) CR_HIP(hipEventRecord(ev0, stream));
     const HamJob *a_jobs = d_jobs;
     uint32_t a_nj = njobs, a_rows = lds_rows;

@@ -63,11 +63,26 @@ extern "C" cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *r
     job.stream = req->stream;
     job.nbins = need;
     job.item_begin = 0;
+    // Work assignment.  The queue balances waves that advance at different
+    // rates (a static assignment measured about 25 % slower on launches of
+    // 0.5 ms and more), but each queue step is a device-scope atomic on one
+    // counter, serialized at about 12.5 ns, one per item plus one per wave:
+    // 40-150 us per launch.  Launches estimated below 100 us of work
+    // therefore run with a static assignment.  Measured rates (words/s):
+    // exhaustive 7.5e12, 32-bit range/stride 1.9e12, 64-bit 0.55e12, Monte
+    // Carlo 1.4e12 / 0.57e12, lists about half the stride rates.
+    double rate = req->k > 32 ? 0.55e12 : 1.9e12;
+    if (req->source == CR_HAM_RANGE && req->first == 0) rate = 7.5e12;
+    if (req->source == CR_HAM_MC) rate = req->k > 32 ? 0.57e12 : 1.4e12;
+    if (req->source == CR_HAM_LIST) rate *= 0.5;
+    const bool short_launch = (double)req->count / rate < 100e-6;
     {
-        // items of 64 x spl words, spl a power of two in [16, kHamMaxLaneSamples],
-        // about 4 or more per resident wave
+        // items of 64 x spl words, spl a power of two in [16, kHamMaxLaneSamples]:
+        // about 8 or more per resident wave for a static assignment (a fine
+        // balance), 4 or more for the queue
         const uint64_t waves = (uint64_t)hamming_waves_per_cu(need + 1) * ctx->cus;
-        const uint64_t want = (req->count + 64 * 4 * waves - 1) / (64 * 4 * waves);
+        const uint64_t per_wave = short_launch ? 8 : 4;
+        const uint64_t want = (req->count + 64 * per_wave * waves - 1) / (64 * per_wave * waves);
         uint64_t spl = 16;
         while (spl < want && spl < kHamMaxLaneSamples) spl *= 2;
         job.lane_samples = (uint32_t)spl;
@@ -79,19 +94,28 @@ extern "C" cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *r
                               ctx->stream));
         job.list = ctx->table_a.as<uint64_t>();
     }
-    CR_TRY(ctx->hist.reserve(need * sizeof(uint64_t)));
+    // kHamOutCopies line-aligned copies of the histogram, summed here
+    job.out_copies = kHamOutCopies;
+    const uint64_t cstride = ham_copy_stride(need);
+    const uint64_t entries = cstride * job.out_copies;
+    CR_TRY(ctx->hist.reserve(entries * sizeof(uint64_t)));
     CR_TRY(ctx->counters.reserve(sizeof(uint64_t)));
     CR_TRY(ctx->jobs.reserve(sizeof(HamJob)));
     job.out = ctx->hist.as<unsigned long long>();
-    CR_TRY(fill_zero_u64(job.out, need, ctx->stream));
+    CR_TRY(fill_zero_u64(job.out, entries, ctx->stream));
     CR_HIP(hipMemcpyAsync(ctx->jobs.ptr, &job, sizeof(job), hipMemcpyHostToDevice, ctx->stream));
     CR_TRY(launch_hamming(ctx->jobs.as<HamJob>(), 1, job.items, req->k > 32, ctx->counters.as<unsigned long long>(),
-                          need + 1, ctx->cus, ctx->stream, kernel_ms ? ctx->ev0 : nullptr,
+                          short_launch, need + 1, ctx->cus, ctx->stream, kernel_ms ? ctx->ev0 : nullptr,
                           kernel_ms ? ctx->ev1 : nullptr));
-    CR_TRY(ctx->host.reserve(need * sizeof(uint64_t)));
-    CR_HIP(hipMemcpyAsync(ctx->host.ptr, job.out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    CR_TRY(ctx->host.reserve(entries * sizeof(uint64_t)));
+    CR_HIP(hipMemcpyAsync(ctx->host.ptr, job.out, entries * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     CR_HIP(hipStreamSynchronize(ctx->stream));
-    memcpy(hist, ctx->host.ptr, need * sizeof(uint64_t));
+    const uint64_t *copies = static_cast<const uint64_t *>(ctx->host.ptr);
+    for (uint32_t b = 0; b < need; ++b) {
+        uint64_t s = 0;
+        for (uint32_t c = 0; c < job.out_copies; ++c) s += copies[c * cstride + b];
+        hist[b] = s;
+    }
     if (kernel_ms) {
         float ms = 0;
         CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

@@ -130,9 +130,7 @@ __global__ __launch_bounds__(64) void rect_kernel(uint64_t A, uint64_t row0, uin
     uint64_t pending = 0;
     bool dirty = false;
 
-    for (;;) {
-        const uint64_t t = next_item(counter);
-        if (t >= total_items) break;
+    for (uint64_t t = first_item(); t < total_items; t = following_item(counter, t)) {
         // item = (group of G row blocks, column chunk)
         const uint64_t rbg = uniform64(t / CC), cc = uniform64(t - (t / CC) * CC);
         const uint64_t j0 = cc * CW;

@@ -261,11 +261,11 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
     }
     CR_HIP(hipEventRecord(evm, stream));
     if (!h32.empty())
-        CR_TRY(launch_hamming(ham_jobs.as<HamJob>(), (uint32_t)h32.size(), t32, false, ctr + 1, lds_ham, ctx->cus,
-                              stream, nullptr, nullptr));
-    if (!h64.empty())
-        CR_TRY(launch_hamming(ham_jobs.as<HamJob>() + h32.size(), (uint32_t)h64.size(), t64, true, ctr + 2, lds_ham,
+        CR_TRY(launch_hamming(ham_jobs.as<HamJob>(), (uint32_t)h32.size(), t32, false, ctr + 1, false, lds_ham,
                               ctx->cus, stream, nullptr, nullptr));
+    if (!h64.empty())
+        CR_TRY(launch_hamming(ham_jobs.as<HamJob>() + h32.size(), (uint32_t)h64.size(), t64, true, ctr + 2, false,
+                              lds_ham, ctx->cus, stream, nullptr, nullptr));
     CR_HIP(hipEventRecord(ev1, stream));
     CR_HIP(hipStreamSynchronize(stream));
     float ms = 0, pms = 0, hms = 0;
