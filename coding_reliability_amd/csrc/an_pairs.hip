@@ -181,7 +181,7 @@ __global__ __launch_bounds__(64) void pair_kernel(const PairJob *__restrict__ jo
             if (pair_prefix(mid, NC, Q) <= tl) lo = mid; else hi = mid - 1;
         }
         const uint64_t b = uniform64(lo);
-        const uint64_t c = b / Q + (tl - pair_prefix(b, NC, Q));
+        const uint64_t c = (Q == 1 ? b : b / Q) + (tl - pair_prefix(b, NC, Q));
         const uint64_t CW = Q * RB;
         const uint64_t row0 = b * RB, row_end = row0 + RB;
         uint64_t j0 = c * CW;
@@ -192,7 +192,8 @@ __global__ __launch_bounds__(64) void pair_kernel(const PairJob *__restrict__ jo
         j1 = uniform64(j1);
 
         if (jb != cur || (pending + (j1 - j0)) * R >= 0xFFFFFFFFull) {
-            if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, jobs[cur].out);
+            if (cur != 0xFFFFFFFFu)
+                flush_rows(lds, 0, jobs[cur].nbins, hist_copy(jobs[cur].out, jobs[cur].nbins, jobs[cur].out_copies));
             cur = jb;
             pending = 0;
         }
@@ -216,7 +217,8 @@ __global__ __launch_bounds__(64) void pair_kernel(const PairJob *__restrict__ jo
                 pairs_w2gen<R>(lane_col, rw, A, jr0, j1);
         }
     }
-    if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, jobs[cur].out);
+    if (cur != 0xFFFFFFFFu)
+        flush_rows(lds, 0, jobs[cur].nbins, hist_copy(jobs[cur].out, jobs[cur].nbins, jobs[cur].out_copies));
 }
 
 template <int R>

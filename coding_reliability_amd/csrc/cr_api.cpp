@@ -3,6 +3,7 @@
 #include <cstdio>
 #include <cstring>
 
+#include "cr_device.h"
 #include "cr_internal.h"
 
 namespace cr {
@@ -57,14 +58,16 @@ void HostBuffer::release() {
     bytes = 0;
 }
 
-// Tuning for one pair job when the caller leaves a knob at 0: enough items
-// (about 16 per resident wave) to balance the persistent waves, rows per lane
-// large enough to amortise the per-item set-up.
 // Single-code launches below this many pairs run with a static assignment.
 constexpr long double kStaticPairs = 1e9;
 
+// Tuning for one pair job when the caller leaves a knob at 0: enough items
+// (about 16 per resident wave) to balance the persistent waves, rows per lane
+// large enough to amortise the per-item set-up.
 static void auto_pair_tuning(uint64_t N, uint32_t resident_waves, uint32_t &R, uint32_t &Q) {
-    if (R == 0) R = N >= (1ull << 16) ? 8 : (N >= (1ull << 11) ? 4 : 2);
+    // rows per lane: small codes favour more, smaller row blocks (measured on
+    // single-code launches: R = 2 up to 2^14 words, 4 at 2^15, 8 from 2^16)
+    if (R == 0) R = N >= (1ull << 16) ? 8 : (N >= (1ull << 15) ? 4 : 2);
     if (Q == 0) {
         const uint64_t RB = 64ull * R;
         const uint64_t T = (N + RB - 1) / RB;
@@ -207,27 +210,35 @@ cr_status cr_an_pairs_tuned(cr_context *ctx, uint64_t A, uint64_t num_words, uin
     job.nbins = need;
     pair_job_geometry(job, R, Q);
 
-    // device: histogram bins then the work-queue head, cleared by one fill;
-    // host (pinned): the job descriptor at 0, the histogram from 256
+    // device: the histogram (as kHistCopies line-aligned copies when many
+    // short-lived waves flush it) then the work-queue head, cleared by one
+    // fill; host (pinned): the job descriptor at 0, the copies from 256
     constexpr size_t kHostResult = 256;
     static_assert(sizeof(PairJob) <= kHostResult, "job staging");
-    CR_TRY(ctx->hist.reserve((need + 1) * sizeof(uint64_t)));
+    job.out_copies = cfg.allow_static ? dev::kHistCopies : 1;
+    const uint64_t cstride = dev::hist_copy_stride(need);
+    const uint64_t entries = job.out_copies > 1 ? cstride * job.out_copies : need;
+    CR_TRY(ctx->hist.reserve((entries + 1) * sizeof(uint64_t)));
     CR_TRY(ctx->jobs.reserve(sizeof(PairJob)));
-    CR_TRY(ctx->host.reserve(kHostResult + need * sizeof(uint64_t)));
+    CR_TRY(ctx->host.reserve(kHostResult + entries * sizeof(uint64_t)));
     job.out = ctx->hist.as<unsigned long long>();
-    unsigned long long *queue = job.out + need;
+    unsigned long long *queue = job.out + entries;
     job.item_begin = 0;
     memcpy(ctx->host.ptr, &job, sizeof(job));
-    CR_TRY(fill_zero_u64(job.out, need + 1, ctx->stream));
+    CR_TRY(fill_zero_u64(job.out, entries + 1, ctx->stream));
     CR_HIP(hipMemcpyAsync(ctx->jobs.ptr, ctx->host.ptr, sizeof(PairJob), hipMemcpyHostToDevice, ctx->stream));
     cfg.zero_counter = false;
     CR_TRY(launch_pairs(ctx->jobs.as<PairJob>(), 1, job.items, queue, cfg, ctx->cus, ctx->stream,
                         kernel_ms ? ctx->ev0 : nullptr, kernel_ms ? ctx->ev1 : nullptr));
-    char *res = ctx->host.as<char>() + kHostResult;
-    CR_HIP(hipMemcpyAsync(res, job.out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    const uint64_t *res = reinterpret_cast<const uint64_t *>(ctx->host.as<char>() + kHostResult);
+    CR_HIP(hipMemcpyAsync((void *)res, job.out, entries * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     CR_HIP(hipStreamSynchronize(ctx->stream));
     memset(raw, 0, nbins * sizeof(uint64_t));
-    memcpy(raw, res, need * sizeof(uint64_t));
+    for (uint32_t b = 0; b < need; ++b) {
+        uint64_t s = 0;
+        for (uint32_t c = 0; c < job.out_copies; ++c) s += res[c * cstride + b];
+        raw[b] = s;
+    }
     if (kernel_ms) {
         float ms = 0;
         CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

@@ -52,6 +52,18 @@ __device__ inline void flush_rows(uint32_t *lds, uint32_t first, uint32_t nbins,
     __syncthreads();
 }
 
+// Histogram copies: a histogram of nbins entries may be kept as `copies`
+// copies (a power of two; 0 means 1), hist_copy_stride(nbins) entries apart,
+// each on whole 128-byte lines; wave b flushes into copy b mod copies and the
+// host sums the copies.  Many waves flushing one histogram otherwise queue
+// their atomics on its few cache lines.
+constexpr uint32_t kHistCopies = 32;
+__host__ __device__ inline uint64_t hist_copy_stride(uint32_t nbins) { return ((uint64_t)nbins + 15) & ~15ull; }
+__device__ __forceinline__ unsigned long long *hist_copy(unsigned long long *out, uint32_t nbins, uint32_t copies) {
+    const uint32_t c = copies > 1 ? (blockIdx.x & (copies - 1)) : 0;
+    return out + (uint64_t)c * hist_copy_stride(nbins);
+}
+
 // Work-queue head: lane 0 takes the next item, every lane sees it.
 __device__ __forceinline__ uint64_t next_item(unsigned long long *counter) {
     uint64_t t = 0;

@@ -76,12 +76,13 @@ struct PairJob {
     uint32_t Q;           // chunk width in row blocks
     uint32_t mode;
     uint32_t nbins;       // bins flushed to out
-    uint32_t pad;
+    uint32_t out_copies;  // copies of the histogram at out (power of two; 0 = 1), hist_copy_stride apart
 };
 static_assert(sizeof(PairJob) == 80, "PairJob layout");
 
-// sum_{x=0}^{b-1} floor(x/Q)
+// sum_{x=0}^{b-1} floor(x/Q); Q = 1 (every small job) without a division
 __host__ __device__ inline uint64_t pair_floor_sum(uint64_t b, uint64_t Q) {
+    if (Q == 1) return b * (b - 1) / 2;
     const uint64_t q = b / Q, r = b - q * Q;
     return Q * (q * (q - (q > 0 ? 1 : 0)) / 2) + r * q;
 }
@@ -145,15 +146,10 @@ struct HamJob {
     uint64_t masks[7];         // parity masks truncated to k bits, zero when unused
     uint32_t source, k, stream, nbins;
     uint32_t lane_samples;     // samples per lane per work item (power of two in [16, kHamMaxLaneSamples])
-    uint32_t out_copies;       // copies of the histogram at out (power of two; 0 = 1), ham_copy_stride apart
+    uint32_t out_copies;       // copies of the histogram at out (power of two; 0 = 1), hist_copy_stride apart
 };
 static_assert(sizeof(HamJob) == 152, "HamJob layout");
 constexpr uint32_t kHamMaxLaneSamples = 4096;
-// Histogram copies of the standalone Hamming entry point: the flush atomics of
-// its waves land on 32 sets of cache lines.
-constexpr uint32_t kHamOutCopies = 32;
-// Entries between two copies of an nbins histogram: whole 128-byte lines.
-__host__ __device__ inline uint64_t ham_copy_stride(uint32_t nbins) { return ((uint64_t)nbins + 15) & ~15ull; }
 
 // Parity masks of the (N,k) extended Hamming codes, truncated to k bits;
 // returns the number of non-overall parity bits.

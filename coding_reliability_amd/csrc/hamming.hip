@@ -182,13 +182,8 @@ __device__ __forceinline__ void seq_item(uint32_t *lane_col, const uint8_t *tab,
     }
 }
 
-// A job's histogram is out_copies copies (a power of two; 0 means 1) of
-// ham_copy_stride(nbins) entries each; wave b flushes into copy b mod
-// out_copies, which spreads the flush atomics of many waves over several cache
-// lines instead of queueing them on the few lines of one histogram.
 __device__ __forceinline__ unsigned long long *copy_out(const HamJob &job) {
-    const uint32_t c = job.out_copies > 1 ? (blockIdx.x & (job.out_copies - 1)) : 0;
-    return job.out + (uint64_t)c * ham_copy_stride(job.nbins);
+    return hist_copy(job.out, job.nbins, job.out_copies);
 }
 
 __device__ __forceinline__ uint32_t find_job(const HamJob *jobs, uint32_t njobs, uint64_t t, uint32_t cur) {

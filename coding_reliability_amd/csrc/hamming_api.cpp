@@ -3,6 +3,7 @@
 #include <cstring>
 #include <vector>
 
+#include "cr_device.h"
 #include "cr_internal.h"
 
 namespace cr {
@@ -94,9 +95,9 @@ extern "C" cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *r
                               ctx->stream));
         job.list = ctx->table_a.as<uint64_t>();
     }
-    // kHamOutCopies line-aligned copies of the histogram, summed here
-    job.out_copies = kHamOutCopies;
-    const uint64_t cstride = ham_copy_stride(need);
+    // dev::kHistCopies line-aligned copies of the histogram, summed here
+    job.out_copies = dev::kHistCopies;
+    const uint64_t cstride = dev::hist_copy_stride(need);
     const uint64_t entries = cstride * job.out_copies;
     CR_TRY(ctx->hist.reserve(entries * sizeof(uint64_t)));
     CR_TRY(ctx->counters.reserve(sizeof(uint64_t)));
