@@ -1,0 +1,209 @@
+// Exhaustive AN distance distribution by the Walsh-Hadamard spectrum of the
+// code (gfx950).
+//
+// Same result as the pair kernel (raw[d] = #{i<j : popcount(A*i ^ A*j) = d},
+// the quantity dancoding_shared accumulates, distance_distribution/src/
+// an_coding.cu:50-105), computed without visiting the 2^(2k-1) pairs.  For the
+// code C = {A*i : i < 2^k} in {0,1}^n with indicator f and Walsh-Hadamard
+// transform F(S) = sum_x f(x) (-1)^(S.x):
+//   D(d) = #{(x, y) in C^2 : |x ^ y| = d} = 2^-n sum_w K_d(w) W(w),
+//   W(w) = sum_{|S| = w} F(S)^2,
+// with K_d the Krawtchouk polynomials of length n (the autocorrelation of f
+// has transform F^2; summing it over the weight-d shell gives K_d).
+//
+// For odd A the low k bits x_lo = A*i mod 2^k run over every residue once
+// (i = x_lo * A^-1 mod 2^k), so with x = (h(x_lo), x_lo), S = (s, S_lo):
+//   F(s, S_lo) = WHT_k[ (-1)^(s . h(x_lo)) ](S_lo),
+// one length-2^k transform of a +-1 sequence per s < 2^(n-k).  The transform
+// runs in passes of 8 bits over a device buffer of int32 (k = 16: two passes,
+// k = 24: three); the last pass squares and sums F^2 by weight |s| + |S_lo|
+// exactly (lo/hi 32-bit halves in 64-bit per-workgroup slots).  The host
+// applies the Krawtchouk transform in 128-bit integers.
+#include "cr_device.h"
+#include "cr_internal.h"
+
+namespace cr {
+namespace {
+
+using namespace dev;
+
+constexpr uint32_t kThreads = 256;  // 4 waves; thread t: column t & 63, quarter p = t >> 6
+constexpr uint32_t kRows = 256;     // transform length of a pass (8 bits)
+constexpr uint32_t kCols = 64;      // contiguous columns (or transforms) per tile
+constexpr uint32_t kExch = 4 * 16 * kCols;  // exchange buffer (ints): one round of 16 rows per wave
+
+// In-register butterflies over the 6 low bits of the register index.
+__device__ __forceinline__ void wht64(int32_t (&v)[64]) {
+#pragma unroll
+    for (int h = 1; h < 64; h <<= 1)
+#pragma unroll
+        for (int j = 0; j < 64; ++j)
+            if ((j & h) == 0) {
+                const int32_t a = v[j], b = v[j + h];
+                v[j] = a + b;
+                v[j + h] = a - b;
+            }
+}
+
+// Row bits 6 and 7 (the quarter p) of a 256-row transform whose rows
+// 64 p + j (j < 64) sit in thread quarter p's registers after wht64.  Four
+// rounds through E (16 rows per wave each, conflict free: the lane is the
+// column) leave rows 64 m + 16 r + 4 p + l in w[16 r + 4 l + m], and the
+// butterflies over m finish the transform.
+__device__ __forceinline__ void exchange_m(int32_t (&w)[64], int32_t *E, uint32_t c, uint32_t p) {
+    int32_t o[64];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) E[(p * 16 + jj) * kCols + c] = w[16 * r + jj];
+        __syncthreads();
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+#pragma unroll
+            for (int m = 0; m < 4; ++m) o[16 * r + 4 * l + m] = E[(m * 16 + 4 * p + l) * kCols + c];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int g = 0; g < 16; ++g) {
+        const int32_t *q = o + 4 * g;
+        const int32_t a0 = q[0] + q[1], a1 = q[0] - q[1], a2 = q[2] + q[3], a3 = q[2] - q[3];
+        w[4 * g] = a0 + a2;
+        w[4 * g + 2] = a0 - a2;
+        w[4 * g + 1] = a1 + a3;
+        w[4 * g + 3] = a1 - a3;
+    }
+}
+
+// Row held in w[i] after exchange_m, for thread quarter p.
+__device__ __forceinline__ uint32_t held_row(int i, uint32_t p) {
+    return 64u * (i & 3) + 16u * (i >> 4) + 4u * p + ((i >> 2) & 3);
+}
+
+// Pass 1: generates the +-1 sequence of chunk s = s0 + blockIdx.y and
+// transforms bits 0..7 of the index.  Tile: 64 transforms (index bits 8..)
+// of 256 contiguous elements, thread column c = transform.  Element x_lo of
+// the chunk holds (-1)^parity(s & h(x_lo)), h(x_lo) = (A * (x_lo * Ainv mod
+// 2^k)) >> k, advanced incrementally along x_lo: i += Ainv (mod 2^k),
+// h += tc - A * wrap, where A * Ainv = 1 + tc * 2^k.
+__global__ __launch_bounds__(kThreads) void spec_gen(int32_t *__restrict__ X, uint32_t k, uint64_t A,
+                                                     uint32_t Ainv, uint64_t tc, uint64_t s0) {
+    __shared__ int32_t E[kExch];
+    const uint32_t t = threadIdx.x, c = t & 63, p = t >> 6;
+    const uint64_t s = s0 + blockIdx.y;
+    const uint32_t mask = (1u << k) - 1;
+    const uint32_t g = blockIdx.x * kCols + c;  // transform: index bits 8..
+    const uint32_t x0 = (g << 8) | (p << 6);    // first of this thread's 64 consecutive x_lo
+    uint32_t i = (uint32_t)((uint64_t)x0 * Ainv) & mask;
+    uint64_t h = (A * (uint64_t)i) >> k;
+    int32_t v[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+        v[j] = 1 - 2 * (int32_t)(__popcll(h & s) & 1);
+        const uint32_t ni = i + Ainv;  // < 2^25: k <= 24
+        const bool wrap = ni > mask;
+        i = ni & mask;
+        h = h + tc - (wrap ? A : 0);
+    }
+    wht64(v);
+    exchange_m(v, E, c, p);
+    int32_t *out = X + ((uint64_t)blockIdx.y << k) + ((uint64_t)g << 8);
+#pragma unroll
+    for (int q = 0; q < 64; ++q) out[held_row(q, p)] = v[q];
+}
+
+// Element index of (row, column) in tile tau of a pass over bits
+// [tbit, tbit + 8): idx = tau_hi << (tbit + 8) | row << tbit | tau_lo << 6 | col.
+__device__ __forceinline__ uint64_t pass_index(uint32_t tau, uint32_t tbit, uint32_t row, uint32_t col) {
+    const uint32_t lo_bits = tbit - 6;
+    const uint64_t tau_lo = tau & ((1u << lo_bits) - 1), tau_hi = tau >> lo_bits;
+    return (tau_hi << (tbit + 8)) | ((uint64_t)row << tbit) | (tau_lo << 6) | col;
+}
+
+// Loads the 256 x 64 tile (rows strided by 2^tbit, 64 contiguous columns)
+// and transforms its rows; w then holds rows held_row(i, p).
+__device__ __forceinline__ void pass_rows(const int32_t *chunk, uint32_t tau, uint32_t tbit, int32_t *E,
+                                          int32_t (&w)[64]) {
+    const uint32_t t = threadIdx.x, c = t & 63, p = t >> 6;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) w[j] = chunk[pass_index(tau, tbit, p * 64 + j, c)];
+    wht64(w);
+    exchange_m(w, E, c, p);
+}
+
+// Middle pass: bits [tbit, tbit + 8) of chunk blockIdx.y, in place (every
+// read of a tile precedes the workgroup barriers that precede its writes).
+__global__ __launch_bounds__(kThreads) void spec_mid(int32_t *__restrict__ X, uint32_t k, uint32_t tbit) {
+    __shared__ int32_t E[kExch];
+    const uint32_t t = threadIdx.x, c = t & 63, p = t >> 6;
+    int32_t *chunk = X + ((uint64_t)blockIdx.y << k);
+    int32_t w[64];
+    pass_rows(chunk, blockIdx.x, tbit, E, w);
+#pragma unroll
+    for (int q = 0; q < 64; ++q) chunk[pass_index(blockIdx.x, tbit, held_row(q, p), c)] = w[q];
+}
+
+// Last pass: bits [tbit, tbit + 8) of every chunk of the batch (tiles
+// blockIdx.x, blockIdx.x + gridDim.x, ... of nchunks * tiles_per_chunk), then
+// W[|s| + |S_lo|] += F^2.  The weight of held_row(i, p) is |p| + |i >> 4| +
+// |(i >> 2) & 3| + |i & 3| over the tile's base |s| + |tau| + |col|, so each
+// thread sums into 7 registers per tile; the workgroup adds them to LDS bins
+// (low and high 32-bit halves of F^2 apart: a half-sum per tile is below
+// 2^38), and at the end to its own slot (2 * nbins uint64 at slots +
+// blockIdx.x * 2 * nbins, read-modify-write: one workgroup per slot).
+__global__ __launch_bounds__(kThreads) void spec_last(const int32_t *__restrict__ X, uint32_t k, uint32_t tbit,
+                                                      uint64_t s0, uint32_t nchunks, uint32_t tiles_per_chunk,
+                                                      unsigned long long *__restrict__ slots, uint32_t nbins) {
+    __shared__ int32_t E[kExch];
+    __shared__ unsigned long long bins[2 * 64];
+    const uint32_t t = threadIdx.x, c = t & 63, p = t >> 6;
+    for (uint32_t b = t; b < 2 * 64; b += kThreads) bins[b] = 0;
+    __syncthreads();
+    const uint64_t ntiles = (uint64_t)nchunks * tiles_per_chunk;
+    const uint32_t lo_bits = tbit - 6;
+    for (uint64_t q = blockIdx.x; q < ntiles; q += gridDim.x) {
+        const uint32_t ch = (uint32_t)(q / tiles_per_chunk), tau = (uint32_t)(q - (uint64_t)ch * tiles_per_chunk);
+        int32_t w[64];
+        pass_rows(X + ((uint64_t)ch << k), tau, tbit, E, w);
+        unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < 64; ++i) {
+            const int64_t f = w[i];
+            acc[__builtin_popcount(i >> 4) + __builtin_popcount((i >> 2) & 3) + __builtin_popcount(i & 3)] +=
+                (unsigned long long)(f * f);
+        }
+        const uint32_t base = (uint32_t)__popcll(s0 + ch) + __popc(tau & ((1u << lo_bits) - 1)) +
+                              __popc(tau >> lo_bits) + __popc(c) + __popc(p);
+#pragma unroll
+        for (int o = 0; o < 7; ++o) {
+            atomicAdd(&bins[2 * (base + o)], acc[o] & 0xFFFFFFFFull);
+            atomicAdd(&bins[2 * (base + o) + 1], acc[o] >> 32);
+        }
+    }
+    __syncthreads();
+    unsigned long long *slot = slots + (uint64_t)blockIdx.x * 2 * nbins;
+    for (uint32_t b = t; b < 2 * nbins; b += kThreads) slot[b] += bins[b];
+}
+
+}  // namespace
+
+size_t spectral_tile_count(uint32_t k) { return (size_t)1 << (k - 14); }
+
+cr_status launch_spectral_batch(int32_t *X, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc, uint64_t s0,
+                                uint32_t nchunks, unsigned long long *slots, uint32_t nslots, uint32_t nbins,
+                                hipStream_t stream) {
+    if (k != 16 && k != 24) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u not 16 or 24", k);
+    if (nbins > 64) return fail(CR_ERR_UNSUPPORTED, "spectral path: %u weights", nbins);
+    const uint32_t tiles = (uint32_t)spectral_tile_count(k);
+    hipLaunchKernelGGL(spec_gen, dim3(tiles, nchunks), dim3(kThreads), 0, stream, X, k, A, Ainv, tc, s0);
+    CR_HIP(hipGetLastError());
+    if (k == 24) {
+        hipLaunchKernelGGL(spec_mid, dim3(tiles, nchunks), dim3(kThreads), 0, stream, X, k, 8u);
+        CR_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(spec_last, dim3(nslots), dim3(kThreads), 0, stream, (const int32_t *)X, k, k - 8, s0, nchunks,
+                       tiles, slots, nbins);
+    CR_HIP(hipGetLastError());
+    return CR_OK;
+}
+
+}  // namespace cr

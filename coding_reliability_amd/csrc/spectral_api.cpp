@@ -1,0 +1,127 @@
+// Host side of the spectral exhaustive AN histogram (spectral.hip): batching
+// of the 2^(n-k) transforms, the exact weight sums W(w) and the Krawtchouk
+// transform D(d) = 2^-n sum_w K_d(w) W(w).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "cr_device.h"
+#include "cr_internal.h"
+
+namespace cr {
+
+size_t spectral_tile_count(uint32_t k);
+cr_status launch_spectral_batch(int32_t *X, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc, uint64_t s0,
+                                uint32_t nchunks, unsigned long long *slots, uint32_t nslots, uint32_t nbins,
+                                hipStream_t stream);
+
+namespace {
+
+using i128 = __int128;
+using u128 = unsigned __int128;
+
+uint32_t bit_length(uint64_t x) { return x ? 64 - (uint32_t)__builtin_clzll(x) : 0; }
+
+// Inverse of an odd a modulo 2^32 (Newton: each step doubles the correct bits).
+uint32_t inverse_mod_2_32(uint32_t a) {
+    uint32_t x = a;  // correct to 3 bits for odd a
+    for (int i = 0; i < 5; ++i) x *= 2u - a * x;
+    return x;
+}
+
+int64_t binom(uint32_t n, uint32_t r) {
+    if (r > n) return 0;
+    int64_t b = 1;
+    for (uint32_t i = 1; i <= r; ++i) b = b * (int64_t)(n - r + i) / (int64_t)i;  // exact at every step
+    return b;
+}
+
+// K_d(w) for length n: sum_j (-1)^j C(w, j) C(n - w, d - j).
+i128 krawtchouk(uint32_t n, uint32_t d, uint32_t w) {
+    i128 s = 0;
+    for (uint32_t j = 0; j <= d && j <= w; ++j) {
+        const i128 t = (i128)binom(w, j) * binom(n - w, d - j);
+        s += (j & 1) ? -t : t;
+    }
+    return s;
+}
+
+}  // namespace
+}  // namespace cr
+
+using namespace cr;
+
+extern "C" cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t k, uint64_t *raw, uint32_t nbins,
+                                          double *kernel_ms) {
+    clear_error();
+    if (!ctx || !raw) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (A == 0) return fail(CR_ERR_INVALID_ARGUMENT, "A must be >= 1");
+    if (k != 16 && k != 24) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u (16 or 24 supported)", k);
+    const uint64_t N = 1ull << k;
+    if ((unsigned __int128)(N - 1) * A >> 64) return fail(CR_ERR_UNSUPPORTED, "codewords (2^k-1)*A exceed 64 bits");
+    const uint32_t need = pair_max_distance(A, N) + 1;
+    if (nbins < need) return fail(CR_ERR_INVALID_ARGUMENT, "nbins=%u but distances reach %u", nbins, need - 1);
+    uint64_t Ao = A;
+    while ((Ao & 1) == 0) Ao >>= 1;
+    if (Ao >> 32) return fail(CR_ERR_UNSUPPORTED, "spectral path: odd part of A above 2^32");
+    const uint32_t n = std::max<uint32_t>(bit_length(Ao * (N - 1)), k);
+    // 2^(n-k) transforms of 2^k: beyond n = 2k that is more work than the
+    // 2^(2k-1) pairs themselves
+    if (n > 48 || n > 2 * k)
+        return fail(CR_ERR_UNSUPPORTED, "spectral path: %u-bit codewords at k=%u (at most min(2k, 48))", n, k);
+    const uint32_t nhi = n - k, nw = n + 1;  // weights 0..n
+    const uint32_t mask = (uint32_t)(N - 1);
+    const uint32_t Ainv = inverse_mod_2_32((uint32_t)Ao) & mask;
+    const uint64_t tc = (Ao * Ainv - 1) >> k;  // Ao * Ainv = 1 + tc * 2^k (below 2^56)
+    CR_HIP(hipSetDevice(ctx->device));
+
+    // chunks of 2^k int32 per launch: about 1 GiB of transforms in flight
+    const uint64_t chunks = 1ull << nhi;
+    const uint64_t chunk_bytes = N * sizeof(int32_t);
+    const uint64_t B = std::min<uint64_t>(chunks, std::max<uint64_t>(1, (1ull << 30) / chunk_bytes));
+    const uint32_t nslots = (uint32_t)ctx->cus * 4;
+    const uint64_t slot_entries = (uint64_t)nslots * 2 * nw;
+    CR_TRY(ctx->table_a.reserve(B * chunk_bytes));
+    CR_TRY(ctx->hist.reserve(slot_entries * sizeof(uint64_t)));
+    CR_TRY(ctx->host.reserve(slot_entries * sizeof(uint64_t)));
+    unsigned long long *slots = ctx->hist.as<unsigned long long>();
+    CR_TRY(fill_zero_u64(slots, slot_entries, ctx->stream));
+    if (kernel_ms) CR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    for (uint64_t s0 = 0; s0 < chunks; s0 += B) {
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(B, chunks - s0);
+        CR_TRY(launch_spectral_batch(ctx->table_a.as<int32_t>(), k, Ao, Ainv, tc, s0, nb, slots, nslots, nw,
+                                     ctx->stream));
+    }
+    if (kernel_ms) CR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    CR_HIP(hipMemcpyAsync(ctx->host.ptr, slots, slot_entries * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    CR_HIP(hipStreamSynchronize(ctx->stream));
+
+    // W(w) = sum over slots of lo + hi * 2^32 (exact in 128 bits: W <= 2^(n+k))
+    const uint64_t *sl = ctx->host.as<uint64_t>();
+    std::vector<u128> W(nw, 0);
+    for (uint32_t q = 0; q < nslots; ++q)
+        for (uint32_t w = 0; w < nw; ++w)
+            W[w] += (u128)sl[(uint64_t)q * 2 * nw + 2 * w] + ((u128)sl[(uint64_t)q * 2 * nw + 2 * w + 1] << 32);
+    // D(d) = 2^-n sum_w K_d(w) W(w); |terms| < 2^(n+k+45+6) <= 2^123
+    memset(raw, 0, nbins * sizeof(uint64_t));
+    for (uint32_t d = 0; d < nw; ++d) {
+        i128 s = 0;
+        for (uint32_t w = 0; w < nw; ++w) s += krawtchouk(n, d, w) * (i128)W[w];
+        const i128 unit = (i128)1 << n;
+        if (s < 0 || s % unit != 0) return fail(CR_ERR_DEVICE, "spectral sums inconsistent at distance %u", d);
+        const i128 D = s / unit;
+        if (d == 0) {
+            if (D != (i128)N) return fail(CR_ERR_DEVICE, "spectral sums inconsistent at distance 0");
+            continue;
+        }
+        if (D & 1) return fail(CR_ERR_DEVICE, "odd ordered-pair count at distance %u", d);
+        raw[d] = (uint64_t)(D / 2);
+    }
+    if (kernel_ms) {
+        float ms = 0;
+        CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *kernel_ms = ms;
+    }
+    return CR_OK;
+}

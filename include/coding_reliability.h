@@ -79,7 +79,7 @@ cr_status cr_an_pairs(cr_context *ctx, uint64_t A, uint64_t num_words, uint64_t 
                       double *kernel_ms);
 
 /* Tuning knobs for the pair kernel; 0 = automatic: 8 rows per lane from 2^16
- * words (4 from 2^11, else 2) and column chunks giving about 16 items per
+ * words (4 from 2^15, else 2) and column chunks giving about 16 items per
  * resident wave (the settings measured fastest for single codes; batches are
  * tuned by cr_autotune_batch). */
 typedef struct cr_pair_tuning {
@@ -90,6 +90,17 @@ typedef struct cr_pair_tuning {
 } cr_pair_tuning;
 cr_status cr_an_pairs_tuned(cr_context *ctx, uint64_t A, uint64_t num_words, uint64_t *raw, uint32_t nbins,
                             const cr_pair_tuning *tuning, double *kernel_ms);
+
+/* Same raw[] as cr_an_pairs with num_words = 2^k, computed from the
+ * Walsh-Hadamard spectrum of the code instead of its pairs: for the odd part
+ * A' of A (the distances of A*i and A'*i agree) and n = bit length of
+ * A'*(2^k-1), one length-2^k transform per s < 2^(n-k), O(2^n * k) work
+ * against the 2^(2k-1) pairs of dancoding_shared
+ * (distance_distribution/src/an_coding.cu:50-105), exact integer arithmetic.
+ * Supported: k = 16 or 24 and n <= 2k (CR_ERR_UNSUPPORTED otherwise).
+ * kernel_ms (may be NULL) receives the device time of all the launches. */
+cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t k, uint64_t *raw, uint32_t nbins,
+                               double *kernel_ms);
 
 /* --- reference grid approximation ------------------------------------------
  * Kernel-level histogram of run_ancoding_grid

@@ -1,0 +1,85 @@
+"""GPU parity of the spectral exhaustive AN histogram (cr_an_pairs_spectral)
+against the reference's recorded k=16 and k=24 result files, its golden
+vectors, and the pair kernel bit for bit."""
+import pytest
+
+from conftest import golden, padded_eq
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import coding_reliability_amd as cra
+    c = cra.Context(0)
+    yield c
+    c.close()
+
+
+def report(k, raw, cc):
+    return O.an_report(k, raw, cc)
+
+
+def test_k16_recorded_runs(ctx):
+    runs = [r for r in golden("an_exhaustive_runs.json")["runs"] if r["n"] == 16]
+    assert len(runs) == 79
+    done = {}
+    for r in runs:
+        if r["A"] not in done:
+            done[r["A"]], _ = ctx.an_pairs_spectral(r["A"], 16)
+        assert report(16, done[r["A"]], len(r["counts"])) == r["counts"], r["src"]
+
+
+def test_k24_recorded_runs(ctx):
+    # results/runtimes/*/ancoding_gpu_A*_n24_*.csv and results/As/ancoding_gpu_A*_n24_*.csv:
+    # 30- to 40-bit codewords
+    runs = [r for r in golden("an_exhaustive_runs.json")["runs"] if r["n"] == 24]
+    assert len(runs) == 23
+    done = {}
+    for r in runs:
+        if r["A"] not in done:
+            done[r["A"]], ms = ctx.an_pairs_spectral(r["A"], 24)
+            print(f"k=24 A={r['A']}: {ms:.1f} ms", flush=True)
+        assert report(24, done[r["A"]], len(r["counts"])) == r["counts"], r["src"]
+
+
+def test_k24_golden_vectors(ctx, solutions):
+    for A in (61, 641):
+        raw, _ = ctx.an_pairs_spectral(A, 24)
+        assert padded_eq(report(24, raw, 31 if A == 61 else 35), solutions[f"solution_an24_A{A}"])
+
+
+@pytest.mark.parametrize("A", [1, 3, 61, 64 * 61, 641, 32771, 58659, 65521, 65535, 3 << 20])
+def test_k16_matches_pair_kernel(ctx, A):
+    # even A (3 << 20: odd part 3), A = 1, and 32-bit codewords (n = 2k) included
+    raw_s, _ = ctx.an_pairs_spectral(A, 16)
+    raw_p, _ = ctx.an_pairs(A, 1 << 16)
+    assert raw_s == raw_p
+
+
+def test_k24_headline_code_matches_pair_kernel(ctx):
+    # A = 58659: 40-bit codewords, 2^16 transforms of length 2^24
+    raw_s, ms_s = ctx.an_pairs_spectral(58659, 24)
+    raw_p, ms_p = ctx.an_pairs(58659, 1 << 24)
+    print(f"k=24 A=58659: spectral {ms_s:.1f} ms, pair kernel {ms_p:.1f} ms", flush=True)
+    assert raw_s == raw_p
+    N = 1 << 24
+    assert sum(raw_s) == N * (N - 1) // 2
+
+
+def test_unsupported_and_invalid(ctx):
+    import coding_reliability_amd as cra
+    with pytest.raises(cra.CRError) as e:
+        ctx.an_pairs_spectral(61, 20)
+    assert e.value.status == 2  # CR_ERR_UNSUPPORTED
+    with pytest.raises(cra.CRError) as e:
+        ctx.an_pairs_spectral(65539, 16)  # 65539 * 65535 has 33 bits: more transforms than pairs
+    assert e.value.status == 2  # CR_ERR_UNSUPPORTED
+    with pytest.raises(cra.CRError) as e:
+        ctx.an_pairs_spectral((1 << 33) + 1, 24)
+    assert e.value.status == 2
+    with pytest.raises(cra.CRError):
+        ctx.an_pairs_spectral(0, 16)
+    with pytest.raises(cra.CRError):
+        ctx.an_pairs_spectral(61, 16, nbins=5)
