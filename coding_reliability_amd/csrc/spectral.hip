@@ -83,32 +83,37 @@ __device__ __forceinline__ uint32_t held_row(int i, uint32_t p) {
 // transforms bits 0..7 of the index.  Tile: 64 transforms (index bits 8..)
 // of 256 contiguous elements, thread column c = transform.  Element x_lo of
 // the chunk holds (-1)^parity(s & h(x_lo)), h(x_lo) = (A * (x_lo * Ainv mod
-// 2^k)) >> k, advanced incrementally along x_lo: i += Ainv (mod 2^k),
-// h += tc - A * wrap, where A * Ainv = 1 + tc * 2^k.
-__global__ __launch_bounds__(kThreads) void spec_gen(int32_t *__restrict__ X, uint32_t k, uint64_t A,
-                                                     uint32_t Ainv, uint64_t tc, uint64_t s0) {
-    __shared__ int32_t E[kExch];
+// 2^k)) >> k < 2^(n-k) <= 2^24, advanced incrementally along x_lo:
+// i += Ainv (mod 2^k), h += tc - A * wrap (mod 2^32), where A * Ainv = 1 +
+// tc * 2^k.  The transformed tile is staged in LDS (transform c, row r at
+// c * 256 + (r ^ c): conflict free both ways) and written row by row.
+__global__ __launch_bounds__(kThreads) void spec_gen(int32_t *__restrict__ X, uint32_t k, uint32_t A,
+                                                     uint32_t Ainv, uint32_t tc, uint32_t s0) {
+    __shared__ int32_t E[kCols * kRows];  // 64 KB; the exchange uses its first kExch entries
     const uint32_t t = threadIdx.x, c = t & 63, p = t >> 6;
-    const uint64_t s = s0 + blockIdx.y;
+    const uint32_t s = s0 + blockIdx.y;
     const uint32_t mask = (1u << k) - 1;
     const uint32_t g = blockIdx.x * kCols + c;  // transform: index bits 8..
     const uint32_t x0 = (g << 8) | (p << 6);    // first of this thread's 64 consecutive x_lo
     uint32_t i = (uint32_t)((uint64_t)x0 * Ainv) & mask;
-    uint64_t h = (A * (uint64_t)i) >> k;
+    uint32_t h = (uint32_t)(((uint64_t)A * i) >> k);
     int32_t v[64];
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
-        v[j] = 1 - 2 * (int32_t)(__popcll(h & s) & 1);
+        v[j] = 1 - 2 * (int32_t)(__popc(h & s) & 1);
         const uint32_t ni = i + Ainv;  // < 2^25: k <= 24
         const bool wrap = ni > mask;
         i = ni & mask;
-        h = h + tc - (wrap ? A : 0);
+        h = h + tc - (wrap ? A : 0u);
     }
     wht64(v);
     exchange_m(v, E, c, p);
-    int32_t *out = X + ((uint64_t)blockIdx.y << k) + ((uint64_t)g << 8);
 #pragma unroll
-    for (int q = 0; q < 64; ++q) out[held_row(q, p)] = v[q];
+    for (int q = 0; q < 64; ++q) E[c * kRows + (held_row(q, p) ^ c)] = v[q];
+    __syncthreads();
+    int32_t *out = X + ((uint64_t)blockIdx.y << k) + ((uint64_t)blockIdx.x * kCols << 8);
+#pragma unroll 8
+    for (uint32_t r = 0; r < kCols; ++r) out[r * kRows + t] = E[r * kRows + (t ^ r)];
 }
 
 // Element index of (row, column) in tile tau of a pass over bits
@@ -194,7 +199,10 @@ cr_status launch_spectral_batch(int32_t *X, uint32_t k, uint64_t A, uint32_t Ain
     if (k != 16 && k != 24) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u not 16 or 24", k);
     if (nbins > 64) return fail(CR_ERR_UNSUPPORTED, "spectral path: %u weights", nbins);
     const uint32_t tiles = (uint32_t)spectral_tile_count(k);
-    hipLaunchKernelGGL(spec_gen, dim3(tiles, nchunks), dim3(kThreads), 0, stream, X, k, A, Ainv, tc, s0);
+    if ((A >> 32) || (tc >> 32) || (s0 + nchunks) > (1ull << 32))
+        return fail(CR_ERR_UNSUPPORTED, "spectral path: multiplier or transform count beyond 32 bits");
+    hipLaunchKernelGGL(spec_gen, dim3(tiles, nchunks), dim3(kThreads), 0, stream, X, k, (uint32_t)A, Ainv,
+                       (uint32_t)tc, (uint32_t)s0);
     CR_HIP(hipGetLastError());
     if (k == 24) {
         hipLaunchKernelGGL(spec_mid, dim3(tiles, nchunks), dim3(kThreads), 0, stream, X, k, 8u);
