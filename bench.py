@@ -247,25 +247,59 @@ REFERENCE_RECORDED = {
 }
 
 
+def run_spectral(ctx, A: int, k: int, raw_pairs: list, pair_kernel_s: float) -> dict:
+    """The headline histogram again, from the code's Walsh-Hadamard spectrum
+    (cr_an_pairs_spectral, the path cr_an_pairs takes for this code): one
+    untimed call, one timed call, checked equal to the pair kernel's."""
+    ctx.an_pairs_spectral(A, k)
+    t0 = time.perf_counter()
+    raw, ms = ctx.an_pairs_spectral(A, k)
+    dt = time.perf_counter() - t0
+    assert raw == raw_pairs, "spectral histogram differs from the pair kernel's"
+    N = 1 << k
+    Ao = A // (A & -A)  # odd part: the same distances
+    n = max((Ao * (N - 1)).bit_length(), k)
+    return {
+        "method": "Walsh-Hadamard spectrum of the code: 2^(n-k) transforms of length 2^k, exact integer sums, "
+                  "Krawtchouk transform on the host",
+        "seconds": round(dt, 4),
+        "device_seconds": round(ms / 1e3, 4),
+        "equivalent_gpairs_per_s": round(N * (N - 1) / 2 / dt / 1e9, 1),
+        "speedup_vs_pair_kernel": round(pair_kernel_s / dt, 2),
+        "transforms": 1 << (n - k),
+        "transform_length": N,
+        "bit_exact_vs_pair_kernel": True,
+    }
+
+
 def run_vs_reference(ctx) -> dict:
     """The workloads the reference records timings for, on this device, each
     checked (complete histograms; batched = per-code results)."""
     from coding_reliability_amd import workloads as WL
     out = {}
 
-    def entry(key, secs, extra=None):
+    def entry(key, secs, extra=None, out_key=None):
         ref, src = REFERENCE_RECORDED[key]
         e = {"seconds": round(secs, 4), "reference_seconds": ref, "reference_source": src,
              "speedup_vs_reference": round(ref / secs, 1)}
         e.update(extra or {})
-        out[key] = e
-        log(f"{key}: {secs:.3f} s (reference {ref} s)")
+        out[out_key or key] = e
+        log(f"{out_key or key}: {secs:.4f} s (reference {ref} s)")
 
     for A, key in ((61, "an_A61_k24"), (32771, "an_A32771_k24")):
         N = 1 << 24
         raw, ms = ctx.an_pairs(A, N)
         assert sum(raw) == N * (N - 1) // 2
-        entry(key, ms / 1e3, {"gpairs_per_s": round(N * (N - 1) / 2 / (ms / 1e3) / 1e9, 1)})
+        entry(key, ms / 1e3, {"gpairs_per_s": round(N * (N - 1) / 2 / (ms / 1e3) / 1e9, 1), "method": "pair kernel"})
+        # the same histogram from the code's spectrum (what cr_an_pairs picks)
+        ctx.an_pairs_spectral(A, 24)
+        t0 = time.perf_counter()
+        raw_s, ms_s = ctx.an_pairs_spectral(A, 24)
+        dt = time.perf_counter() - t0
+        assert raw_s == raw
+        entry(key, dt, {"device_seconds": round(ms_s / 1e3, 4),
+                        "method": "Walsh-Hadamard spectrum (cr_an_pairs_spectral), bit-exact against the pair kernel"},
+              out_key=key + "_spectral")
     for k, m, key in ((32, 1001, "grid1d_A61_k32_m1001"), (40, 101, "grid1d_A61_k40_m101")):
         h, ms = ctx.an_grid_ref(1, k, m, 0, 61)
         entry(key, ms / 1e3, {"gpairs_per_s": round(sum(h) / (ms / 1e3) / 1e9, 1)})
@@ -423,6 +457,7 @@ def main() -> int:
     ap.add_argument("--rows-per-lane", type=int, default=0)
     ap.add_argument("--chunk-blocks", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-per-launch comparison")
+    ap.add_argument("--no-spectral", action="store_true", help="skip the spectral computation of the headline histogram")
     ap.add_argument("--served", action="store_true", help="also run the served request mix")
     ap.add_argument("--no-kernels", action="store_true", help="skip the other kernels' measurements")
     ap.add_argument("--kernels-only", action="store_true", help="only the other kernels (for counter runs)")
@@ -510,6 +545,12 @@ def main() -> int:
         },
     }
 
+    spectral = None
+    if not args.no_spectral:
+        log("spectral ...")
+        spectral = run_spectral(ctx, A, k, raw, avg_kernel_s)
+        log(f"spectral: {spectral['seconds']} s, {spectral['equivalent_gpairs_per_s']} Gpairs/s equivalent")
+
     batched = None
     if not args.no_batched:
         log("batched vs per-launch ...")
@@ -551,6 +592,7 @@ def main() -> int:
                                f"all C(2^{k},2) pairs, {width_bits + 1} bins",
                    "A": A, "k": k, "pairs_per_step": pairs_per_step, "parallelism": "single GPU"},
         "roofline": roofline,
+        "spectral": spectral,
         "cpu_baseline": cpu,
         "batched": batched,
         "kernels": kernels,

@@ -142,6 +142,16 @@ class Context:
         check(lib().cr_an_pairs_tuned(self._h, A, num_words, _u64p(out), nbins, C.byref(t), C.byref(ms)))
         return [int(x) for x in out], ms.value
 
+    def an_pairs_auto(self, A: int, num_words: int, nbins: int | None = None) -> tuple[list[int], float]:
+        """The same histogram as an_pairs through cr_an_pairs, which picks the
+        spectral path for 2^16 / 2^24 words when it is estimated cheaper."""
+        if nbins is None:
+            nbins = an_bins(A, num_words)
+        out = np.zeros(nbins, dtype=np.uint64)
+        ms = C.c_double(0)
+        check(lib().cr_an_pairs(self._h, A, num_words, _u64p(out), nbins, C.byref(ms)))
+        return [int(x) for x in out], ms.value
+
     def an_pairs_spectral(self, A: int, k: int, nbins: int | None = None) -> tuple[list[int], float]:
         """Same raw histogram as an_pairs(A, 2^k), from the Walsh-Hadamard
         spectrum of the code (cr_an_pairs_spectral), and the device time in ms."""

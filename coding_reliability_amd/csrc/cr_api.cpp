@@ -249,6 +249,9 @@ cr_status cr_an_pairs_tuned(cr_context *ctx, uint64_t A, uint64_t num_words, uin
 
 cr_status cr_an_pairs(cr_context *ctx, uint64_t A, uint64_t num_words, uint64_t *raw, uint32_t nbins,
                       double *kernel_ms) {
+    // the same histogram from the code's spectrum when that is clearly cheaper
+    uint32_t k = 0;
+    if (spectral_preferred(A, num_words, &k)) return cr_an_pairs_spectral(ctx, A, k, raw, nbins, kernel_ms);
     return cr_an_pairs_tuned(ctx, A, num_words, raw, nbins, nullptr, kernel_ms);
 }
 

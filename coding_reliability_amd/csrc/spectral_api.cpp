@@ -2,6 +2,7 @@
 // of the 2^(n-k) transforms, the exact weight sums W(w) and the Krawtchouk
 // transform D(d) = 2^-n sum_w K_d(w) W(w).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -46,7 +47,37 @@ i128 krawtchouk(uint32_t n, uint32_t d, uint32_t w) {
     return s;
 }
 
+// Odd part of A, codeword bits n and support of the spectral path at k.
+bool spectral_shape(uint64_t A, uint32_t k, uint64_t *Ao, uint32_t *n) {
+    if (A == 0 || (k != 16 && k != 24)) return false;
+    const uint64_t N = 1ull << k;
+    if ((unsigned __int128)(N - 1) * A >> 64) return false;
+    uint64_t a = A;
+    while ((a & 1) == 0) a >>= 1;
+    if (a >> 32) return false;
+    const uint32_t bits = std::max<uint32_t>(bit_length(a * (N - 1)), k);
+    // 2^(n-k) transforms of 2^k: beyond n = 2k that is more work than the
+    // 2^(2k-1) pairs themselves
+    if (bits > 48 || bits > 2 * k) return false;
+    *Ao = a;
+    *n = bits;
+    return true;
+}
+
 }  // namespace
+
+bool spectral_preferred(uint64_t A, uint64_t num_words, uint32_t *k) {
+    if (num_words == 0 || (num_words & (num_words - 1))) return false;
+    const uint32_t kk = (uint32_t)__builtin_ctzll(num_words);
+    uint64_t Ao = 0;
+    uint32_t n = 0;
+    if (!spectral_shape(A, kk, &Ao, &n)) return false;
+    const double spectral_s = std::ldexp(2.8e-12, (int)n) + 50e-6;
+    const double pair_s = std::ldexp(1.0, 2 * (int)kk - 1) / 9.6e12;
+    *k = kk;
+    return spectral_s < 0.8 * pair_s;
+}
+
 }  // namespace cr
 
 using namespace cr;
@@ -61,14 +92,11 @@ extern "C" cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t 
     if ((unsigned __int128)(N - 1) * A >> 64) return fail(CR_ERR_UNSUPPORTED, "codewords (2^k-1)*A exceed 64 bits");
     const uint32_t need = pair_max_distance(A, N) + 1;
     if (nbins < need) return fail(CR_ERR_INVALID_ARGUMENT, "nbins=%u but distances reach %u", nbins, need - 1);
-    uint64_t Ao = A;
-    while ((Ao & 1) == 0) Ao >>= 1;
-    if (Ao >> 32) return fail(CR_ERR_UNSUPPORTED, "spectral path: odd part of A above 2^32");
-    const uint32_t n = std::max<uint32_t>(bit_length(Ao * (N - 1)), k);
-    // 2^(n-k) transforms of 2^k: beyond n = 2k that is more work than the
-    // 2^(2k-1) pairs themselves
-    if (n > 48 || n > 2 * k)
-        return fail(CR_ERR_UNSUPPORTED, "spectral path: %u-bit codewords at k=%u (at most min(2k, 48))", n, k);
+    uint64_t Ao = 0;
+    uint32_t n = 0;
+    if (!spectral_shape(A, k, &Ao, &n))
+        return fail(CR_ERR_UNSUPPORTED, "spectral path: A=%llu at k=%u (odd part below 2^32, codewords of at most "
+                    "min(2k, 48) bits)", (unsigned long long)A, k);
     const uint32_t nhi = n - k, nw = n + 1;  // weights 0..n
     const uint32_t mask = (uint32_t)(N - 1);
     const uint32_t Ainv = inverse_mod_2_32((uint32_t)Ao) & mask;

@@ -74,7 +74,11 @@ cr_status cr_device_memory(cr_context *ctx, uint64_t *free_bytes, uint64_t *tota
  * (distance_distribution/src/an_coding.cu:50-105) before run_ancoding doubles it
  * (an_coding.cu:287-292).  With A = A0*s and num_words = m it is the
  * integer-stride grid {A0*(i*s) : i < m}.
- * kernel_ms (may be NULL) receives the device time of the pair kernel. */
+ * For num_words = 2^16 or 2^24 the same histogram is computed from the code's
+ * Walsh-Hadamard spectrum (cr_an_pairs_spectral) when that is estimated at
+ * under 80 % of the pair kernel's time (multipliers well below 2^k);
+ * cr_an_pairs_tuned always runs the pair kernel.
+ * kernel_ms (may be NULL) receives the device time of the launches. */
 cr_status cr_an_pairs(cr_context *ctx, uint64_t A, uint64_t num_words, uint64_t *raw, uint32_t nbins,
                       double *kernel_ms);
 

@@ -68,6 +68,22 @@ def test_k24_headline_code_matches_pair_kernel(ctx):
     assert sum(raw_s) == N * (N - 1) // 2
 
 
+@pytest.mark.parametrize("A,k", [(61, 16), (58659, 16), (641, 18)])
+def test_auto_dispatch_matches_pair_kernel(ctx, A, k):
+    # cr_an_pairs takes the spectral path for (61, 16), the pair kernel for
+    # the others; either way the histogram is the pair kernel's
+    raw_a, _ = ctx.an_pairs_auto(A, 1 << k)
+    raw_p, _ = ctx.an_pairs(A, 1 << k)
+    assert raw_a == raw_p
+
+
+def test_auto_dispatch_k24(ctx, solutions):
+    raw, ms = ctx.an_pairs_auto(61, 1 << 24)
+    print(f"cr_an_pairs k=24 A=61: {ms:.1f} ms", flush=True)
+    assert ms < 1000  # the spectral path, not the 15 s pair kernel
+    assert padded_eq(report(24, raw, 31), solutions["solution_an24_A61"])
+
+
 def test_unsupported_and_invalid(ctx):
     import coding_reliability_amd as cra
     with pytest.raises(cra.CRError) as e:
