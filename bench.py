@@ -458,6 +458,8 @@ def main() -> int:
     ap.add_argument("--chunk-blocks", type=int, default=0)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-vs-per-launch comparison")
     ap.add_argument("--no-spectral", action="store_true", help="skip the spectral computation of the headline histogram")
+    ap.add_argument("--spectral-only", nargs=2, type=int, metavar=("A", "K"),
+                    help="one cr_an_pairs_spectral call for (A, K), checked to sum to C(2^K, 2) (for counter runs)")
     ap.add_argument("--served", action="store_true", help="also run the served request mix")
     ap.add_argument("--no-kernels", action="store_true", help="skip the other kernels' measurements")
     ap.add_argument("--kernels-only", action="store_true", help="only the other kernels (for counter runs)")
@@ -487,6 +489,13 @@ def main() -> int:
     peak_tops = info["compute_units"] * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12
     if args.kernels_only:
         print(json.dumps({"kernels": run_kernels(ctx, peak_tops)}), flush=True)
+        ctx.close()
+        return 0
+    if args.spectral_only:
+        sA, sk = args.spectral_only
+        raw, ms = ctx.an_pairs_spectral(sA, sk)
+        assert sum(raw) == (1 << sk) * ((1 << sk) - 1) // 2
+        print(json.dumps({"spectral": {"A": sA, "k": sk, "device_ms": round(ms, 3)}}), flush=True)
         ctx.close()
         return 0
     if args.vs_reference:
