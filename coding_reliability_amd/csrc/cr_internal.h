@@ -153,7 +153,7 @@ constexpr uint32_t kHamMaxLaneSamples = 4096;
 
 // True when num_words = 2^k and cr_an_pairs_spectral supports (A, k) and is
 // estimated at under 80 % of the pair kernel's time (measured rates:
-// 2.8e-12 s per transformed element plus about 50 us per call, against
+// 2.2e-12 s per transformed element plus about 50 us per call, against
 // 9.6e12 pairs/s).
 bool spectral_preferred(uint64_t A, uint64_t num_words, uint32_t *k);
 

@@ -15,8 +15,10 @@
 // (i = x_lo * A^-1 mod 2^k), so with x = (h(x_lo), x_lo), S = (s, S_lo):
 //   F(s, S_lo) = WHT_k[ (-1)^(s . h(x_lo)) ](S_lo),
 // one length-2^k transform of a +-1 sequence per s < 2^(n-k).  The transform
-// runs in passes of 8 bits over a device buffer of int32 (k = 16: two passes,
-// k = 24: three); the last pass squares and sums F^2 by weight |s| + |S_lo|
+// runs in two passes through a device buffer: k = 24, index bits 0-15 in a
+// 1024-thread workgroup (int32 out), then bits 16-23; k = 16, bits 0-7 (int16
+// out, values within +-2^8), then bits 8-15.  The last pass squares and sums
+// F^2 by weight |s| + |S_lo|
 // exactly (lo/hi 32-bit halves in 64-bit per-workgroup slots).  The host
 // applies the Krawtchouk transform in 128-bit integers.
 #include "cr_device.h"
@@ -87,8 +89,9 @@ __device__ __forceinline__ uint32_t held_row(int i, uint32_t p) {
 // i += Ainv (mod 2^k), h += tc - A * wrap (mod 2^32), where A * Ainv = 1 +
 // tc * 2^k.  The transformed tile is staged in LDS (transform c, row r at
 // c * 256 + (r ^ c): conflict free both ways) and written row by row.
-__global__ __launch_bounds__(kThreads) void spec_gen(int32_t *__restrict__ X, uint32_t k, uint32_t A,
-                                                     uint32_t Ainv, uint32_t tc, uint32_t s0) {
+template <class Out>
+__global__ __launch_bounds__(kThreads) void spec_gen(Out *__restrict__ X, uint32_t k, uint32_t A, uint32_t Ainv,
+                                                     uint32_t tc, uint32_t s0) {
     __shared__ int32_t E[kCols * kRows];  // 64 KB; the exchange uses its first kExch entries
     const uint32_t t = threadIdx.x, c = t & 63, p = t >> 6;
     const uint32_t s = s0 + blockIdx.y;
@@ -111,9 +114,88 @@ __global__ __launch_bounds__(kThreads) void spec_gen(int32_t *__restrict__ X, ui
 #pragma unroll
     for (int q = 0; q < 64; ++q) E[c * kRows + (held_row(q, p) ^ c)] = v[q];
     __syncthreads();
-    int32_t *out = X + ((uint64_t)blockIdx.y << k) + ((uint64_t)blockIdx.x * kCols << 8);
+    Out *out = X + ((uint64_t)blockIdx.y << k) + ((uint64_t)blockIdx.x * kCols << 8);
 #pragma unroll 8
-    for (uint32_t r = 0; r < kCols; ++r) out[r * kRows + t] = E[r * kRows + (t ^ r)];
+    for (uint32_t r = 0; r < kCols; ++r) out[r * kRows + t] = (Out)E[r * kRows + (t ^ r)];
+}
+
+// First pass over index bits 0..15 for k = 24: workgroup (G, chunk) of 1024
+// threads transforms the 2^16 elements G * 2^16 + e.  Thread (wave w, lane l)
+// generates e = w * 4096 + l * 64 + j (j < 64) and transforms bits 0-5 in
+// registers; bits 6-11 (the lane) after a transpose within the wave through
+// LDS as int8 (values within +-2^6; four per dword, dword (l, m) of wave w at
+// w * 1024 + l * 16 + (m ^ (l >> 2)): conflict free both ways); bits 12-15
+// (the wave) after an exchange through LDS as int16 (values within +-2^12) in
+// two rounds of 32 rows, thread (w, l) ending with rows 4w .. 4w+3 of every
+// wave; the result (within +-2^16) is written as int32, 256 bytes per row.
+constexpr uint32_t kThreads16 = 1024;
+__global__ __launch_bounds__(kThreads16) void spec_gen16(int32_t *__restrict__ X, uint32_t A, uint32_t Ainv,
+                                                         uint32_t tc, uint32_t s0) {
+    constexpr uint32_t k = 24, mask = (1u << k) - 1;
+    __shared__ int32_t L[16384];  // 64 KB
+    const uint32_t t = threadIdx.x, l = t & 63, w = t >> 6;
+    const uint32_t s = s0 + blockIdx.y;
+    const uint32_t G = blockIdx.x;
+    const uint32_t x0 = (G << 16) | (w << 12) | (l << 6);
+    uint32_t i = (uint32_t)((uint64_t)x0 * Ainv) & mask;
+    uint32_t h = (uint32_t)(((uint64_t)A * i) >> k);
+    int32_t v[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+        v[j] = 1 - 2 * (int32_t)(__popc(h & s) & 1);
+        const uint32_t ni = i + Ainv;
+        const bool wrap = ni > mask;
+        i = ni & mask;
+        h = h + tc - (wrap ? A : 0u);
+    }
+    wht64(v);  // bits 0-5; |v| <= 64
+    // transpose within the wave: element (row l, column j) -> thread (w, j)
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const uint32_t d = (uint32_t)(v[4 * m] & 0xFF) | ((uint32_t)(v[4 * m + 1] & 0xFF) << 8) |
+                           ((uint32_t)(v[4 * m + 2] & 0xFF) << 16) | ((uint32_t)(v[4 * m + 3] & 0xFF) << 24);
+        L[w * 1024 + l * 16 + (m ^ (l >> 2))] = (int32_t)d;
+    }
+    __syncthreads();
+    {
+        const int8_t *L8 = reinterpret_cast<const int8_t *>(L);
+        const uint32_t m = l >> 2, b = l & 3;  // this thread's column j = l: dword m, byte b
+#pragma unroll
+        for (int r = 0; r < 64; ++r) v[r] = L8[4 * (w * 1024 + r * 16 + (m ^ ((uint32_t)r >> 2))) + b];
+    }
+    wht64(v);  // bits 6-11 (v[r]: row r); |v| <= 2^12
+    __syncthreads();
+    // exchange across waves: thread (w, l) gathers rows 4w + rr (rr < 4) of all 16 waves
+    int16_t *L16 = reinterpret_cast<int16_t *>(L);  // [wave][32 rows][64 lanes] per round
+    int32_t u[64];                                   // u[4 * src + rr]
+#pragma unroll
+    for (int hr = 0; hr < 2; ++hr) {
+#pragma unroll
+        for (int rr = 0; rr < 32; ++rr) L16[(w * 32 + rr) * 64 + l] = (int16_t)v[32 * hr + rr];
+        __syncthreads();
+        if ((w >> 3) == (uint32_t)hr) {
+#pragma unroll
+            for (int src = 0; src < 16; ++src)
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) u[4 * src + rr] = L16[(src * 32 + 4 * (w & 7) + rr) * 64 + l];
+        }
+        __syncthreads();
+    }
+    // bits 12-15: butterflies over src (bits 2-5 of the u index)
+#pragma unroll
+    for (int hb = 4; hb < 64; hb <<= 1)
+#pragma unroll
+        for (int q = 0; q < 64; ++q)
+            if ((q & hb) == 0) {
+                const int32_t a = u[q], c2 = u[q + hb];
+                u[q] = a + c2;
+                u[q + hb] = a - c2;
+            }
+    int32_t *out = X + ((uint64_t)blockIdx.y << k) + ((uint64_t)G << 16);
+#pragma unroll
+    for (int src = 0; src < 16; ++src)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) out[src * 4096 + (4 * w + rr) * 64 + l] = u[4 * src + rr];
 }
 
 // Element index of (row, column) in tile tau of a pass over bits
@@ -126,25 +208,14 @@ __device__ __forceinline__ uint64_t pass_index(uint32_t tau, uint32_t tbit, uint
 
 // Loads the 256 x 64 tile (rows strided by 2^tbit, 64 contiguous columns)
 // and transforms its rows; w then holds rows held_row(i, p).
-__device__ __forceinline__ void pass_rows(const int32_t *chunk, uint32_t tau, uint32_t tbit, int32_t *E,
+template <class In>
+__device__ __forceinline__ void pass_rows(const In *chunk, uint32_t tau, uint32_t tbit, int32_t *E,
                                           int32_t (&w)[64]) {
     const uint32_t t = threadIdx.x, c = t & 63, p = t >> 6;
 #pragma unroll
-    for (int j = 0; j < 64; ++j) w[j] = chunk[pass_index(tau, tbit, p * 64 + j, c)];
+    for (int j = 0; j < 64; ++j) w[j] = (int32_t)chunk[pass_index(tau, tbit, p * 64 + j, c)];
     wht64(w);
     exchange_m(w, E, c, p);
-}
-
-// Middle pass: bits [tbit, tbit + 8) of chunk blockIdx.y, in place (every
-// read of a tile precedes the workgroup barriers that precede its writes).
-__global__ __launch_bounds__(kThreads) void spec_mid(int32_t *__restrict__ X, uint32_t k, uint32_t tbit) {
-    __shared__ int32_t E[kExch];
-    const uint32_t t = threadIdx.x, c = t & 63, p = t >> 6;
-    int32_t *chunk = X + ((uint64_t)blockIdx.y << k);
-    int32_t w[64];
-    pass_rows(chunk, blockIdx.x, tbit, E, w);
-#pragma unroll
-    for (int q = 0; q < 64; ++q) chunk[pass_index(blockIdx.x, tbit, held_row(q, p), c)] = w[q];
 }
 
 // Last pass: bits [tbit, tbit + 8) of every chunk of the batch (tiles
@@ -155,7 +226,8 @@ __global__ __launch_bounds__(kThreads) void spec_mid(int32_t *__restrict__ X, ui
 // (low and high 32-bit halves of F^2 apart: a half-sum per tile is below
 // 2^38), and at the end to its own slot (2 * nbins uint64 at slots +
 // blockIdx.x * 2 * nbins, read-modify-write: one workgroup per slot).
-__global__ __launch_bounds__(kThreads) void spec_last(const int32_t *__restrict__ X, uint32_t k, uint32_t tbit,
+template <class In>
+__global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, uint32_t k, uint32_t tbit,
                                                       uint64_t s0, uint32_t nchunks, uint32_t tiles_per_chunk,
                                                       unsigned long long *__restrict__ slots, uint32_t nbins) {
     __shared__ int32_t E[kExch];
@@ -168,7 +240,7 @@ __global__ __launch_bounds__(kThreads) void spec_last(const int32_t *__restrict_
     for (uint64_t q = blockIdx.x; q < ntiles; q += gridDim.x) {
         const uint32_t ch = (uint32_t)(q / tiles_per_chunk), tau = (uint32_t)(q - (uint64_t)ch * tiles_per_chunk);
         int32_t w[64];
-        pass_rows(X + ((uint64_t)ch << k), tau, tbit, E, w);
+        pass_rows<In>(X + ((uint64_t)ch << k), tau, tbit, E, w);
         unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int i = 0; i < 64; ++i) {
@@ -193,23 +265,31 @@ __global__ __launch_bounds__(kThreads) void spec_last(const int32_t *__restrict_
 
 size_t spectral_tile_count(uint32_t k) { return (size_t)1 << (k - 14); }
 
-cr_status launch_spectral_batch(int32_t *X, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc, uint64_t s0,
+cr_status launch_spectral_batch(void *X, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc, uint64_t s0,
                                 uint32_t nchunks, unsigned long long *slots, uint32_t nslots, uint32_t nbins,
                                 hipStream_t stream) {
     if (k != 16 && k != 24) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u not 16 or 24", k);
     if (nbins > 64) return fail(CR_ERR_UNSUPPORTED, "spectral path: %u weights", nbins);
-    const uint32_t tiles = (uint32_t)spectral_tile_count(k);
     if ((A >> 32) || (tc >> 32) || (s0 + nchunks) > (1ull << 32))
         return fail(CR_ERR_UNSUPPORTED, "spectral path: multiplier or transform count beyond 32 bits");
-    hipLaunchKernelGGL(spec_gen, dim3(tiles, nchunks), dim3(kThreads), 0, stream, X, k, (uint32_t)A, Ainv,
-                       (uint32_t)tc, (uint32_t)s0);
-    CR_HIP(hipGetLastError());
+    const uint32_t tiles = (uint32_t)spectral_tile_count(k);
     if (k == 24) {
-        hipLaunchKernelGGL(spec_mid, dim3(tiles, nchunks), dim3(kThreads), 0, stream, X, k, 8u);
+        // X: nchunks * 2^24 int32 after index bits 0-15
+        int32_t *x32 = static_cast<int32_t *>(X);
+        hipLaunchKernelGGL(spec_gen16, dim3(1u << 8, nchunks), dim3(kThreads16), 0, stream, x32, (uint32_t)A, Ainv,
+                           (uint32_t)tc, (uint32_t)s0);
         CR_HIP(hipGetLastError());
+        hipLaunchKernelGGL(spec_last<int32_t>, dim3(nslots), dim3(kThreads), 0, stream, (const int32_t *)x32, k,
+                           16u, s0, nchunks, tiles, slots, nbins);
+    } else {
+        // X: nchunks * 2^16 int16 after index bits 0-7
+        int16_t *x16 = static_cast<int16_t *>(X);
+        hipLaunchKernelGGL(spec_gen<int16_t>, dim3(tiles, nchunks), dim3(kThreads), 0, stream, x16, k, (uint32_t)A,
+                           Ainv, (uint32_t)tc, (uint32_t)s0);
+        CR_HIP(hipGetLastError());
+        hipLaunchKernelGGL(spec_last<int16_t>, dim3(nslots), dim3(kThreads), 0, stream, (const int16_t *)x16, k,
+                           8u, s0, nchunks, tiles, slots, nbins);
     }
-    hipLaunchKernelGGL(spec_last, dim3(nslots), dim3(kThreads), 0, stream, (const int32_t *)X, k, k - 8, s0, nchunks,
-                       tiles, slots, nbins);
     CR_HIP(hipGetLastError());
     return CR_OK;
 }

@@ -12,7 +12,7 @@
 namespace cr {
 
 size_t spectral_tile_count(uint32_t k);
-cr_status launch_spectral_batch(int32_t *X, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc, uint64_t s0,
+cr_status launch_spectral_batch(void *X, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc, uint64_t s0,
                                 uint32_t nchunks, unsigned long long *slots, uint32_t nslots, uint32_t nbins,
                                 hipStream_t stream);
 
@@ -72,7 +72,7 @@ bool spectral_preferred(uint64_t A, uint64_t num_words, uint32_t *k) {
     uint64_t Ao = 0;
     uint32_t n = 0;
     if (!spectral_shape(A, kk, &Ao, &n)) return false;
-    const double spectral_s = std::ldexp(2.8e-12, (int)n) + 50e-6;
+    const double spectral_s = std::ldexp(2.2e-12, (int)n) + 50e-6;
     const double pair_s = std::ldexp(1.0, 2 * (int)kk - 1) / 9.6e12;
     *k = kk;
     return spectral_s < 0.8 * pair_s;
@@ -103,11 +103,11 @@ extern "C" cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t 
     const uint64_t tc = (Ao * Ainv - 1) >> k;  // Ao * Ainv = 1 + tc * 2^k (below 2^56)
     CR_HIP(hipSetDevice(ctx->device));
 
-    // chunks of 2^k int32 per launch: 256 MB of transforms in flight, which
-    // the passes re-read mostly from the 256 MB last-level cache (measured at
-    // k = 24: 3.03 s with 256 MB, 3.31 s with 128 MB, 3.67 s with 1 GB)
+    // chunks of 2^k per launch: 256 MB of transforms in flight, about the
+    // size of the last-level cache (measured at k = 24, A = 58659: 2.36 s with
+    // 256 MB, 2.63 s with 128 MB, 2.67 s with 512 MB)
     const uint64_t chunks = 1ull << nhi;
-    const uint64_t chunk_bytes = N * sizeof(int32_t);
+    const uint64_t chunk_bytes = N * (k == 24 ? sizeof(int32_t) : sizeof(int16_t));
     const uint64_t B = std::min<uint64_t>(chunks, std::max<uint64_t>(1, (1ull << 28) / chunk_bytes));
     const uint32_t nslots = (uint32_t)ctx->cus * 4;
     const uint64_t slot_entries = (uint64_t)nslots * 2 * nw;
@@ -119,7 +119,7 @@ extern "C" cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t 
     if (kernel_ms) CR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     for (uint64_t s0 = 0; s0 < chunks; s0 += B) {
         const uint32_t nb = (uint32_t)std::min<uint64_t>(B, chunks - s0);
-        CR_TRY(launch_spectral_batch(ctx->table_a.as<int32_t>(), k, Ao, Ainv, tc, s0, nb, slots, nslots, nw,
+        CR_TRY(launch_spectral_batch(ctx->table_a.ptr, k, Ao, Ainv, tc, s0, nb, slots, nslots, nw,
                                      ctx->stream));
     }
     if (kernel_ms) CR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
