@@ -15,9 +15,11 @@
 // (i = x_lo * A^-1 mod 2^k), so with x = (h(x_lo), x_lo), S = (s, S_lo):
 //   F(s, S_lo) = WHT_k[ (-1)^(s . h(x_lo)) ](S_lo),
 // one length-2^k transform of a +-1 sequence per s < 2^(n-k).  The transform
-// runs in two passes through a device buffer: k = 24, index bits 0-15 in a
-// 1024-thread workgroup (int32 out), then bits 16-23; k = 16, bits 0-7 (int16
-// out, values within +-2^8), then bits 8-15.  The last pass squares and sums
+// runs in passes through a device buffer: k = 24, index bits 0-15 in a
+// 1024-thread workgroup (int32 out), then bits 16-23; k = 32, the same first
+// pass, bits 16-23 in place, then bits 24-31 with the last two stages in 64
+// bits (values reach 2^32); k = 16, bits 0-7 (int16 out, values within
+// +-2^8), then bits 8-15.  The last pass squares and sums
 // F^2 by weight |s| + |S_lo|
 // exactly (lo/hi 32-bit halves in 64-bit per-workgroup slots).  The host
 // applies the Krawtchouk transform in 128-bit integers.
@@ -52,8 +54,8 @@ __device__ __forceinline__ void wht64(int32_t (&v)[64]) {
 // rounds through E (16 rows per wave each, conflict free: the lane is the
 // column) leave rows 64 m + 16 r + 4 p + l in w[16 r + 4 l + m], and the
 // butterflies over m finish the transform.
-__device__ __forceinline__ void exchange_m(int32_t (&w)[64], int32_t *E, uint32_t c, uint32_t p) {
-    int32_t o[64];
+__device__ __forceinline__ void exchange_gather(const int32_t (&w)[64], int32_t (&o)[64], int32_t *E, uint32_t c,
+                                                uint32_t p) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
 #pragma unroll
@@ -65,6 +67,10 @@ __device__ __forceinline__ void exchange_m(int32_t (&w)[64], int32_t *E, uint32_
             for (int m = 0; m < 4; ++m) o[16 * r + 4 * l + m] = E[(m * 16 + 4 * p + l) * kCols + c];
         __syncthreads();
     }
+}
+__device__ __forceinline__ void exchange_m(int32_t (&w)[64], int32_t *E, uint32_t c, uint32_t p) {
+    int32_t o[64];
+    exchange_gather(w, o, E, c, p);
 #pragma unroll
     for (int g = 0; g < 16; ++g) {
         const int32_t *q = o + 4 * g;
@@ -119,7 +125,7 @@ __global__ __launch_bounds__(kThreads) void spec_gen(Out *__restrict__ X, uint32
     for (uint32_t r = 0; r < kCols; ++r) out[r * kRows + t] = (Out)E[r * kRows + (t ^ r)];
 }
 
-// First pass over index bits 0..15 for k = 24: workgroup (G, chunk) of 1024
+// First pass over index bits 0..15 for k = 24 and 32: workgroup (G, chunk) of 1024
 // threads transforms the 2^16 elements G * 2^16 + e.  Thread (wave w, lane l)
 // generates e = w * 4096 + l * 64 + j (j < 64) and transforms bits 0-5 in
 // registers; bits 6-11 (the lane) after a transpose within the wave through
@@ -129,9 +135,9 @@ __global__ __launch_bounds__(kThreads) void spec_gen(Out *__restrict__ X, uint32
 // two rounds of 32 rows, thread (w, l) ending with rows 4w .. 4w+3 of every
 // wave; the result (within +-2^16) is written as int32, 256 bytes per row.
 constexpr uint32_t kThreads16 = 1024;
-__global__ __launch_bounds__(kThreads16) void spec_gen16(int32_t *__restrict__ X, uint32_t A, uint32_t Ainv,
-                                                         uint32_t tc, uint32_t s0) {
-    constexpr uint32_t k = 24, mask = (1u << k) - 1;
+__global__ __launch_bounds__(kThreads16) void spec_gen16(int32_t *__restrict__ X, uint32_t k, uint32_t A,
+                                                         uint32_t Ainv, uint32_t tc, uint32_t s0) {
+    const uint32_t mask = k >= 32 ? 0xFFFFFFFFu : (1u << k) - 1;
     __shared__ int32_t L[16384];  // 64 KB
     const uint32_t t = threadIdx.x, l = t & 63, w = t >> 6;
     const uint32_t s = s0 + blockIdx.y;
@@ -144,7 +150,7 @@ __global__ __launch_bounds__(kThreads16) void spec_gen16(int32_t *__restrict__ X
     for (int j = 0; j < 64; ++j) {
         v[j] = 1 - 2 * (int32_t)(__popc(h & s) & 1);
         const uint32_t ni = i + Ainv;
-        const bool wrap = ni > mask;
+        const bool wrap = ni > mask || ni < i;  // carry out of 32 bits at k = 32
         i = ni & mask;
         h = h + tc - (wrap ? A : 0u);
     }
@@ -191,7 +197,7 @@ __global__ __launch_bounds__(kThreads16) void spec_gen16(int32_t *__restrict__ X
                 u[q] = a + c2;
                 u[q + hb] = a - c2;
             }
-    int32_t *out = X + ((uint64_t)blockIdx.y << k) + ((uint64_t)G << 16);
+    int32_t *out = X + ((uint64_t)blockIdx.y << k) + ((uint64_t)G << 16);  // |u| <= 2^16
 #pragma unroll
     for (int src = 0; src < 16; ++src)
 #pragma unroll
@@ -218,15 +224,42 @@ __device__ __forceinline__ void pass_rows(const In *chunk, uint32_t tau, uint32_
     exchange_m(w, E, c, p);
 }
 
+// Middle pass for k = 32: index bits [tbit, tbit + 8) of chunk blockIdx.y,
+// int32 in place (values within +-2^24; every read of a tile precedes the
+// workgroup barriers that precede its writes).
+__global__ __launch_bounds__(kThreads) void spec_mid(int32_t *__restrict__ X, uint32_t k, uint32_t tbit) {
+    __shared__ int32_t E[kExch];
+    const uint32_t t = threadIdx.x, c = t & 63, p = t >> 6;
+    int32_t *chunk = X + ((uint64_t)blockIdx.y << k);
+    int32_t w[64];
+    pass_rows<int32_t>(chunk, blockIdx.x, tbit, E, w);
+#pragma unroll
+    for (int q = 0; q < 64; ++q) chunk[pass_index(blockIdx.x, tbit, held_row(q, p), c)] = w[q];
+}
+
+// F^2 into the 32-bit halves lo, hi (|f| <= 2^32: F^2 = 2^64 is hi = 2^32).
+__device__ __forceinline__ void add_square(int64_t f, unsigned long long &lo, unsigned long long &hi) {
+    const unsigned long long a = (unsigned long long)(f < 0 ? -f : f);
+    if (a >> 32) {
+        hi += 1ull << 32;
+    } else {
+        const unsigned long long q = a * a;
+        lo += q & 0xFFFFFFFFull;
+        hi += q >> 32;
+    }
+}
+
 // Last pass: bits [tbit, tbit + 8) of every chunk of the batch (tiles
 // blockIdx.x, blockIdx.x + gridDim.x, ... of nchunks * tiles_per_chunk), then
 // W[|s| + |S_lo|] += F^2.  The weight of held_row(i, p) is |p| + |i >> 4| +
 // |(i >> 2) & 3| + |i & 3| over the tile's base |s| + |tau| + |col|, so each
-// thread sums into 7 registers per tile; the workgroup adds them to LDS bins
-// (low and high 32-bit halves of F^2 apart: a half-sum per tile is below
-// 2^38), and at the end to its own slot (2 * nbins uint64 at slots +
-// blockIdx.x * 2 * nbins, read-modify-write: one workgroup per slot).
-template <class In>
+// thread sums into 7 pairs of registers per tile (low and high 32-bit halves
+// of F^2: a half-sum per tile is below 2^38); the workgroup adds them to LDS
+// bins, and at the end to its own slot (2 * nbins uint64 at slots +
+// blockIdx.x * 2 * nbins, read-modify-write: one workgroup per slot).  WIDE
+// (k = 32): the values reach +-2^32, so the last two butterfly stages run in
+// 64 bits.
+template <class In, bool WIDE>
 __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, uint32_t k, uint32_t tbit,
                                                       uint64_t s0, uint32_t nchunks, uint32_t tiles_per_chunk,
                                                       unsigned long long *__restrict__ slots, uint32_t nbins) {
@@ -239,21 +272,48 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
     const uint32_t lo_bits = tbit - 6;
     for (uint64_t q = blockIdx.x; q < ntiles; q += gridDim.x) {
         const uint32_t ch = (uint32_t)(q / tiles_per_chunk), tau = (uint32_t)(q - (uint64_t)ch * tiles_per_chunk);
+        const In *src = X + ((uint64_t)ch << k);
+        unsigned long long lo[7] = {0, 0, 0, 0, 0, 0, 0}, hi[7] = {0, 0, 0, 0, 0, 0, 0};
         int32_t w[64];
-        pass_rows<In>(X + ((uint64_t)ch << k), tau, tbit, E, w);
-        unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0};
+        if (WIDE) {
 #pragma unroll
-        for (int i = 0; i < 64; ++i) {
-            const int64_t f = w[i];
-            acc[__builtin_popcount(i >> 4) + __builtin_popcount((i >> 2) & 3) + __builtin_popcount(i & 3)] +=
-                (unsigned long long)(f * f);
+            for (int j = 0; j < 64; ++j) w[j] = (int32_t)src[pass_index(tau, tbit, p * 64 + j, c)];
+            wht64(w);  // |w| <= 2^30
+            int32_t o[64];
+            exchange_gather(w, o, E, c, p);
+#pragma unroll
+            for (int g = 0; g < 16; ++g) {
+                const int64_t q0 = o[4 * g], q1 = o[4 * g + 1], q2 = o[4 * g + 2], q3 = o[4 * g + 3];
+                const int64_t a0 = q0 + q1, a1 = q0 - q1, a2 = q2 + q3, a3 = q2 - q3;
+                const int64_t f[4] = {a0 + a2, a1 + a3, a0 - a2, a1 - a3};  // rows m = 0..3
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int o_ = __builtin_popcount(g >> 2) + __builtin_popcount(g & 3) + __builtin_popcount(m);
+                    add_square(f[m], lo[o_], hi[o_]);
+                }
+            }
+        } else {
+            // |F| <= 2^24: a thread's 64 squares stay below 2^54 in one register pair
+            pass_rows<In>(src, tau, tbit, E, w);
+            unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int i = 0; i < 64; ++i) {
+                const int64_t f = w[i];
+                acc[__builtin_popcount(i >> 4) + __builtin_popcount((i >> 2) & 3) + __builtin_popcount(i & 3)] +=
+                    (unsigned long long)(f * f);
+            }
+#pragma unroll
+            for (int o_ = 0; o_ < 7; ++o_) {
+                lo[o_] = acc[o_] & 0xFFFFFFFFull;
+                hi[o_] = acc[o_] >> 32;
+            }
         }
         const uint32_t base = (uint32_t)__popcll(s0 + ch) + __popc(tau & ((1u << lo_bits) - 1)) +
                               __popc(tau >> lo_bits) + __popc(c) + __popc(p);
 #pragma unroll
-        for (int o = 0; o < 7; ++o) {
-            atomicAdd(&bins[2 * (base + o)], acc[o] & 0xFFFFFFFFull);
-            atomicAdd(&bins[2 * (base + o) + 1], acc[o] >> 32);
+        for (int o_ = 0; o_ < 7; ++o_) {
+            atomicAdd(&bins[2 * (base + o_)], lo[o_]);
+            atomicAdd(&bins[2 * (base + o_) + 1], hi[o_]);
         }
     }
     __syncthreads();
@@ -268,27 +328,35 @@ size_t spectral_tile_count(uint32_t k) { return (size_t)1 << (k - 14); }
 cr_status launch_spectral_batch(void *X, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc, uint64_t s0,
                                 uint32_t nchunks, unsigned long long *slots, uint32_t nslots, uint32_t nbins,
                                 hipStream_t stream) {
-    if (k != 16 && k != 24) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u not 16 or 24", k);
+    if (k != 16 && k != 24 && k != 32) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u not 16, 24 or 32", k);
     if (nbins > 64) return fail(CR_ERR_UNSUPPORTED, "spectral path: %u weights", nbins);
     if ((A >> 32) || (tc >> 32) || (s0 + nchunks) > (1ull << 32))
         return fail(CR_ERR_UNSUPPORTED, "spectral path: multiplier or transform count beyond 32 bits");
     const uint32_t tiles = (uint32_t)spectral_tile_count(k);
-    if (k == 24) {
-        // X: nchunks * 2^24 int32 after index bits 0-15
-        int32_t *x32 = static_cast<int32_t *>(X);
-        hipLaunchKernelGGL(spec_gen16, dim3(1u << 8, nchunks), dim3(kThreads16), 0, stream, x32, (uint32_t)A, Ainv,
-                           (uint32_t)tc, (uint32_t)s0);
-        CR_HIP(hipGetLastError());
-        hipLaunchKernelGGL(spec_last<int32_t>, dim3(nslots), dim3(kThreads), 0, stream, (const int32_t *)x32, k,
-                           16u, s0, nchunks, tiles, slots, nbins);
-    } else {
+    if (k == 16) {
         // X: nchunks * 2^16 int16 after index bits 0-7
         int16_t *x16 = static_cast<int16_t *>(X);
         hipLaunchKernelGGL(spec_gen<int16_t>, dim3(tiles, nchunks), dim3(kThreads), 0, stream, x16, k, (uint32_t)A,
                            Ainv, (uint32_t)tc, (uint32_t)s0);
         CR_HIP(hipGetLastError());
-        hipLaunchKernelGGL(spec_last<int16_t>, dim3(nslots), dim3(kThreads), 0, stream, (const int16_t *)x16, k,
-                           8u, s0, nchunks, tiles, slots, nbins);
+        hipLaunchKernelGGL((spec_last<int16_t, false>), dim3(nslots), dim3(kThreads), 0, stream,
+                           (const int16_t *)x16, k, 8u, s0, nchunks, tiles, slots, nbins);
+        CR_HIP(hipGetLastError());
+        return CR_OK;
+    }
+    // X: nchunks * 2^k int32 after index bits 0-15
+    int32_t *x32 = static_cast<int32_t *>(X);
+    hipLaunchKernelGGL(spec_gen16, dim3(1u << (k - 16), nchunks), dim3(kThreads16), 0, stream, x32, k, (uint32_t)A,
+                       Ainv, (uint32_t)tc, (uint32_t)s0);
+    CR_HIP(hipGetLastError());
+    if (k == 24) {
+        hipLaunchKernelGGL((spec_last<int32_t, false>), dim3(nslots), dim3(kThreads), 0, stream,
+                           (const int32_t *)x32, k, 16u, s0, nchunks, tiles, slots, nbins);
+    } else {
+        hipLaunchKernelGGL(spec_mid, dim3(tiles, nchunks), dim3(kThreads), 0, stream, x32, k, 16u);
+        CR_HIP(hipGetLastError());
+        hipLaunchKernelGGL((spec_last<int32_t, true>), dim3(nslots), dim3(kThreads), 0, stream,
+                           (const int32_t *)x32, k, 24u, s0, nchunks, tiles, slots, nbins);
     }
     CR_HIP(hipGetLastError());
     return CR_OK;

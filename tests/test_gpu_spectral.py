@@ -84,10 +84,42 @@ def test_auto_dispatch_k24(ctx, solutions):
     assert padded_eq(report(24, raw, 31), solutions["solution_an24_A61"])
 
 
+@pytest.mark.parametrize("A", [1, 1 << 7])
+def test_k32_closed_form(ctx, A):
+    # all 2^32 words (odd part 1): D(d) = 2^32 C(32, d), so raw[d] = 2^31 C(32, d);
+    # the transform reaches F = 2^32 at S = 0, the 64-bit end of the last pass
+    import math
+    raw, ms = ctx.an_pairs_spectral(A, 32)
+    print(f"k=32 A={A}: {ms:.1f} ms", flush=True)
+    assert raw[0] == 0
+    assert raw[1:33] == [(1 << 31) * math.comb(32, d) for d in range(1, 33)]
+    assert all(x == 0 for x in raw[33:])
+
+
+@pytest.mark.parametrize("A", [61, 3])
+def test_k32_exact_totals(ctx, A):
+    # exact k = 32 distributions (beyond the pair kernel: 2^63 pairs); the
+    # library checks that every Krawtchouk sum divides by 2^n and D(0) = 2^32
+    raw, ms = ctx.an_pairs_spectral(A, 32)
+    print(f"k=32 A={A}: {ms:.1f} ms", flush=True)
+    N = 1 << 32
+    assert sum(raw) == N * (N - 1) // 2
+    n = (A * (N - 1)).bit_length()
+    assert raw[0] == 0 and all(x == 0 for x in raw[n + 1:])
+    # no multiple of an odd A > 1 is a power of two: no distance 1; distance 2
+    # occurs (A = 3: 3 = 2 + 1; A = 61: 2^30 + 1 = 61 * 17602325, below 2^32)
+    assert raw[1] == 0 and raw[2] > 0
+    raw_auto, _ = ctx.an_pairs_auto(A, N)
+    assert raw_auto == raw
+
+
 def test_unsupported_and_invalid(ctx):
     import coding_reliability_amd as cra
     with pytest.raises(cra.CRError) as e:
         ctx.an_pairs_spectral(61, 20)
+    assert e.value.status == 2  # CR_ERR_UNSUPPORTED
+    with pytest.raises(cra.CRError) as e:
+        ctx.an_pairs_spectral(1 << 16 | 1, 32)  # 49-bit codewords
     assert e.value.status == 2  # CR_ERR_UNSUPPORTED
     with pytest.raises(cra.CRError) as e:
         ctx.an_pairs_spectral(65539, 16)  # 65539 * 65535 has 33 bits: more transforms than pairs
