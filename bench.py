@@ -300,6 +300,24 @@ def run_vs_reference(ctx) -> dict:
         entry(key, dt, {"device_seconds": round(ms_s / 1e3, 4),
                         "method": "Walsh-Hadamard spectrum (cr_an_pairs_spectral), bit-exact against the pair kernel"},
               out_key=key + "_spectral")
+    # the exact k = 32 distribution the grid above approximates (spectral path)
+    ctx.an_pairs_spectral(3, 32)
+    t0 = time.perf_counter()
+    raw32, ms32 = ctx.an_pairs_spectral(61, 32)
+    dt = time.perf_counter() - t0
+    exact = [1 << 32] + [2 * x for x in raw32[1:]]
+    assert sum(exact) == 1 << 64
+    grids = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden",
+                                         "an_grid1d_runs.json")))["runs"]
+    errs = {}
+    for m in (1001, 10000):
+        g = [r for r in grids if r["n"] == 32 and r["A"] == 61 and r["m"] == m][0]["counts"]
+        errs[f"recorded_grid_m{m}_max_rel_error"] = round(max(abs(g[d] / exact[d] - 1) for d in range(1, 38)
+                                                              if exact[d] > 0), 6)
+    entry("grid1d_A61_k32_m1001", dt, dict(errs, device_seconds=round(ms32 / 1e3, 4),
+                                           method="exact distribution (spectral path, 64 transforms of 2^32) "
+                                                  "against the reference's m=1001 grid approximation"),
+          out_key="exact_A61_k32")
     for k, m, key in ((32, 1001, "grid1d_A61_k32_m1001"), (40, 101, "grid1d_A61_k40_m101")):
         h, ms = ctx.an_grid_ref(1, k, m, 0, 61)
         entry(key, ms / 1e3, {"gpairs_per_s": round(sum(h) / (ms / 1e3) / 1e9, 1)})

@@ -113,6 +113,25 @@ def test_k32_exact_totals(ctx, A):
     assert raw_auto == raw
 
 
+def test_k32_exact_against_recorded_grid_approximations(ctx):
+    # results/runtimes/*/ancoding_grid1d_*_m{1001,10000}_A61_n32_h6.csv: the
+    # reference's own k=32 approximations converge on the exact distribution
+    raw, _ = ctx.an_pairs_spectral(61, 32)
+    exact = [1 << 32] + [2 * x for x in raw[1:]]
+    assert sum(exact) == 1 << 64
+    runs = golden("an_grid1d_runs.json")["runs"]
+    err, l1 = {}, {}
+    for m in (1001, 10000):
+        g = [r for r in runs if r["n"] == 32 and r["A"] == 61 and r["m"] == m][0]["counts"]
+        rel = {d: abs(g[d] / exact[d] - 1) for d in range(1, 38) if exact[d] > 0}
+        err[m] = max(rel.values())
+        l1[m] = sum(abs(g[d] - exact[d]) for d in range(39)) / (1 << 64)
+        # the bulk (bins 4..33, counts above 3e13) within 1e-3; the tails carry
+        # the grid's sampling noise
+        assert all(rel[d] < 1e-3 for d in range(4, 34)), m
+    assert err[1001] < 0.02 and err[10000] < 0.005 and l1[10000] < l1[1001]
+
+
 def test_unsupported_and_invalid(ctx):
     import coding_reliability_amd as cra
     with pytest.raises(cra.CRError) as e:
