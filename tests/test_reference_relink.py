@@ -48,6 +48,24 @@ def test_cli_exhaustive_matches_recorded_run(tmp_path):
 
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+def test_cli_exhaustive_k32_is_exact(tmp_path):
+    # the reference driver's exhaustive mode at k = 32 (2^63 pairs: out of
+    # reach for pair enumeration) now returns the exact distribution, through
+    # cr_an_pairs's spectral path
+    run_cli(tmp_path, "-a", "61", "-n", "32", "-f")
+    got = read_counts(tmp_path / "ancoding_gpu_A61_n32_h6.csv")
+    import coding_reliability_amd as cra
+    ctx = cra.Context(0)
+    raw, _ = ctx.an_pairs_spectral(61, 32)
+    ctx.close()
+    want = [1 << 32] + [2 * x for x in raw[1:]]
+    n = max(len(got), len(want))
+    assert got + [0] * (n - len(got)) == want + [0] * (n - len(want))
+    assert sum(got) == 1 << 64
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
 def test_cli_grid_matches_recorded_run(tmp_path):
     run_cli(tmp_path, "-a", "61", "-n", "16", "-m", "1001", "-g", "1", "-f")
     got = read_counts(tmp_path / "ancoding_grid1d_gpu_m1001_A61_n16_h6.csv")
