@@ -15,11 +15,12 @@
 // (i = x_lo * A^-1 mod 2^k), so with x = (h(x_lo), x_lo), S = (s, S_lo):
 //   F(s, S_lo) = WHT_k[ (-1)^(s . h(x_lo)) ](S_lo),
 // one length-2^k transform of a +-1 sequence per s < 2^(n-k).  The transform
-// runs in passes through a device buffer: k = 24, index bits 0-15 in a
-// 1024-thread workgroup (int32 out), then bits 16-23; k = 32, the same first
-// pass, bits 16-23 in place, then bits 24-31 with the last two stages in 64
-// bits (values reach 2^32); k = 16, bits 0-7 (int16 out, values within
-// +-2^8), then bits 8-15.  The last pass squares and sums
+// runs in passes through a device buffer: for k >= 17, index bits 0-15 in a
+// 1024-thread workgroup (int32 out), then bits 16-23 in place when k >= 25,
+// then the remaining top bits (8 of them on 256 x 64 tiles, 1..7 with a
+// thread per column); k = 32 runs its last two stages in 64 bits (values
+// reach 2^32); for 14 <= k <= 16, bits 0-7 (int16 out, values within +-2^8),
+// then the rest.  The last pass squares and sums
 // F^2 by weight |s| + |S_lo|
 // exactly (lo/hi 32-bit halves in 64-bit per-workgroup slots).  The host
 // applies the Krawtchouk transform in 128-bit integers.
@@ -237,6 +238,19 @@ __global__ __launch_bounds__(kThreads) void spec_mid(int32_t *__restrict__ X, ui
     for (int q = 0; q < 64; ++q) chunk[pass_index(blockIdx.x, tbit, held_row(q, p), c)] = w[q];
 }
 
+// Adds the workgroup's LDS bins (lo, hi halves per weight) to its slot: four
+// uint64 per weight, (lo sum, carries of it, hi sum, carries of it), so a
+// slot holds 128-bit sums however many launches add to it.  One workgroup per
+// slot and launch: plain read-modify-write.
+__device__ __forceinline__ void flush_slot(const unsigned long long *bins, unsigned long long *slot,
+                                           uint32_t nbins) {
+    for (uint32_t b = threadIdx.x; b < 2 * nbins; b += blockDim.x) {
+        const unsigned long long x = bins[b], old = slot[2 * b];
+        slot[2 * b] = old + x;
+        if (old + x < old) slot[2 * b + 1] += 1;
+    }
+}
+
 // F^2 into the 32-bit halves lo, hi (|f| <= 2^32: F^2 = 2^64 is hi = 2^32).
 __device__ __forceinline__ void add_square(int64_t f, unsigned long long &lo, unsigned long long &hi) {
     const unsigned long long a = (unsigned long long)(f < 0 ? -f : f);
@@ -255,8 +269,8 @@ __device__ __forceinline__ void add_square(int64_t f, unsigned long long &lo, un
 // |(i >> 2) & 3| + |i & 3| over the tile's base |s| + |tau| + |col|, so each
 // thread sums into 7 pairs of registers per tile (low and high 32-bit halves
 // of F^2: a half-sum per tile is below 2^38); the workgroup adds them to LDS
-// bins, and at the end to its own slot (2 * nbins uint64 at slots +
-// blockIdx.x * 2 * nbins, read-modify-write: one workgroup per slot).  WIDE
+// bins, and at the end to its own slot (4 * nbins uint64 at slots +
+// blockIdx.x * 4 * nbins, see flush_slot).  WIDE
 // (k = 32): the values reach +-2^32, so the last two butterfly stages run in
 // 64 bits.
 template <class In, bool WIDE>
@@ -317,8 +331,85 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
         }
     }
     __syncthreads();
-    unsigned long long *slot = slots + (uint64_t)blockIdx.x * 2 * nbins;
-    for (uint32_t b = t; b < 2 * nbins; b += kThreads) slot[b] += bins[b];
+    flush_slot(bins, slots + (uint64_t)blockIdx.x * 4 * nbins, nbins);
+}
+
+// Narrow last pass: the R < 8 top index bits [tbit, tbit + R) of every chunk,
+// tiles of 2^R rows (strided by 2^tbit) x 256 contiguous columns, thread t
+// owning column t and all 2^R rows in registers (no exchange).  Element
+// (row, column) of tile tau is tau_hi << (tbit + R) | row << tbit |
+// tau_lo << 8 | column, tau_lo being tbit - 8 bits.  F^2 is summed as in
+// spec_last, by the popcount of the row over the base |s| + |tau| + |col|.
+template <class In, int R, bool WIDE>
+__global__ __launch_bounds__(kThreads) void spec_last_narrow(const In *__restrict__ X, uint32_t k, uint32_t tbit,
+                                                             uint64_t s0, uint32_t nchunks,
+                                                             uint32_t tiles_per_chunk,
+                                                             unsigned long long *__restrict__ slots,
+                                                             uint32_t nbins) {
+    constexpr int NR = 1 << R;
+    __shared__ unsigned long long bins[2 * 64];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t b = t; b < 2 * 64; b += kThreads) bins[b] = 0;
+    __syncthreads();
+    const uint64_t ntiles = (uint64_t)nchunks * tiles_per_chunk;
+    const uint32_t lo_bits = tbit - 8;
+    for (uint64_t q = blockIdx.x; q < ntiles; q += gridDim.x) {
+        const uint32_t ch = (uint32_t)(q / tiles_per_chunk), tau = (uint32_t)(q - (uint64_t)ch * tiles_per_chunk);
+        const uint64_t tau_lo = tau & ((1u << lo_bits) - 1), tau_hi = tau >> lo_bits;
+        const In *src = X + ((uint64_t)ch << k) + (tau_hi << (tbit + R)) + (tau_lo << 8) + t;
+        int64_t v[NR];
+#pragma unroll
+        for (int r = 0; r < NR; ++r) v[r] = (int64_t)src[(uint64_t)r << tbit];
+#pragma unroll
+        for (int h = 1; h < NR; h <<= 1)
+#pragma unroll
+            for (int j = 0; j < NR; ++j)
+                if ((j & h) == 0) {
+                    const int64_t a = v[j], b = v[j + h];
+                    v[j] = a + b;
+                    v[j + h] = a - b;
+                }
+        unsigned long long lo[R + 1], hi[R + 1];
+#pragma unroll
+        for (int o_ = 0; o_ <= R; ++o_) lo[o_] = hi[o_] = 0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            if (WIDE) {
+                add_square(v[r], lo[__builtin_popcount(r)], hi[__builtin_popcount(r)]);
+            } else {
+                const unsigned long long sq = (unsigned long long)(v[r] * v[r]);  // < 2^63: k <= 31
+                lo[__builtin_popcount(r)] += sq & 0xFFFFFFFFull;
+                hi[__builtin_popcount(r)] += sq >> 32;
+            }
+        }
+        const uint32_t base = (uint32_t)__popcll(s0 + ch) + __popc(tau) + __popc(t);
+#pragma unroll
+        for (int o_ = 0; o_ <= R; ++o_) {
+            atomicAdd(&bins[2 * (base + o_)], lo[o_]);
+            atomicAdd(&bins[2 * (base + o_) + 1], hi[o_]);
+        }
+    }
+    __syncthreads();
+    flush_slot(bins, slots + (uint64_t)blockIdx.x * 4 * nbins, nbins);
+}
+
+template <class In, bool WIDE>
+cr_status launch_narrow(uint32_t R, const In *X, uint32_t k, uint32_t tbit, uint64_t s0, uint32_t nchunks,
+                        unsigned long long *slots, uint32_t nslots, uint32_t nbins, hipStream_t stream) {
+    const uint32_t tiles = 1u << (k - R - 8);
+#define CR_NARROW(RR)                                                                                            \
+    case RR:                                                                                                     \
+        hipLaunchKernelGGL((spec_last_narrow<In, RR, WIDE>), dim3(nslots), dim3(kThreads), 0, stream, X, k, tbit, \
+                           s0, nchunks, tiles, slots, nbins);                                                    \
+        break;
+    switch (R) {
+        CR_NARROW(1) CR_NARROW(2) CR_NARROW(3) CR_NARROW(4) CR_NARROW(5) CR_NARROW(6) CR_NARROW(7)
+    default:
+        return fail(CR_ERR_UNSUPPORTED, "spectral path: narrow pass of %u bits", R);
+    }
+#undef CR_NARROW
+    CR_HIP(hipGetLastError());
+    return CR_OK;
 }
 
 }  // namespace
@@ -328,17 +419,19 @@ size_t spectral_tile_count(uint32_t k) { return (size_t)1 << (k - 14); }
 cr_status launch_spectral_batch(void *X, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc, uint64_t s0,
                                 uint32_t nchunks, unsigned long long *slots, uint32_t nslots, uint32_t nbins,
                                 hipStream_t stream) {
-    if (k != 16 && k != 24 && k != 32) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u not 16, 24 or 32", k);
+    if (k < 14 || k > 32) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u outside [14, 32]", k);
     if (nbins > 64) return fail(CR_ERR_UNSUPPORTED, "spectral path: %u weights", nbins);
     if ((A >> 32) || (tc >> 32) || (s0 + nchunks) > (1ull << 32))
         return fail(CR_ERR_UNSUPPORTED, "spectral path: multiplier or transform count beyond 32 bits");
     const uint32_t tiles = (uint32_t)spectral_tile_count(k);
-    if (k == 16) {
-        // X: nchunks * 2^16 int16 after index bits 0-7
+    if (k <= 16) {
+        // X: nchunks * 2^k int16 after index bits 0-7 (tiles of 64 transforms: k >= 14)
         int16_t *x16 = static_cast<int16_t *>(X);
         hipLaunchKernelGGL(spec_gen<int16_t>, dim3(tiles, nchunks), dim3(kThreads), 0, stream, x16, k, (uint32_t)A,
                            Ainv, (uint32_t)tc, (uint32_t)s0);
         CR_HIP(hipGetLastError());
+        if (k < 16)
+            return launch_narrow<int16_t, false>(k - 8, x16, k, 8, s0, nchunks, slots, nslots, nbins, stream);
         hipLaunchKernelGGL((spec_last<int16_t, false>), dim3(nslots), dim3(kThreads), 0, stream,
                            (const int16_t *)x16, k, 8u, s0, nchunks, tiles, slots, nbins);
         CR_HIP(hipGetLastError());
@@ -349,15 +442,18 @@ cr_status launch_spectral_batch(void *X, uint32_t k, uint64_t A, uint32_t Ainv, 
     hipLaunchKernelGGL(spec_gen16, dim3(1u << (k - 16), nchunks), dim3(kThreads16), 0, stream, x32, k, (uint32_t)A,
                        Ainv, (uint32_t)tc, (uint32_t)s0);
     CR_HIP(hipGetLastError());
+    if (k < 24) return launch_narrow<int32_t, false>(k - 16, x32, k, 16, s0, nchunks, slots, nslots, nbins, stream);
     if (k == 24) {
         hipLaunchKernelGGL((spec_last<int32_t, false>), dim3(nslots), dim3(kThreads), 0, stream,
                            (const int32_t *)x32, k, 16u, s0, nchunks, tiles, slots, nbins);
-    } else {
-        hipLaunchKernelGGL(spec_mid, dim3(tiles, nchunks), dim3(kThreads), 0, stream, x32, k, 16u);
         CR_HIP(hipGetLastError());
-        hipLaunchKernelGGL((spec_last<int32_t, true>), dim3(nslots), dim3(kThreads), 0, stream,
-                           (const int32_t *)x32, k, 24u, s0, nchunks, tiles, slots, nbins);
+        return CR_OK;
     }
+    hipLaunchKernelGGL(spec_mid, dim3(tiles, nchunks), dim3(kThreads), 0, stream, x32, k, 16u);
+    CR_HIP(hipGetLastError());
+    if (k < 32) return launch_narrow<int32_t, false>(k - 24, x32, k, 24, s0, nchunks, slots, nslots, nbins, stream);
+    hipLaunchKernelGGL((spec_last<int32_t, true>), dim3(nslots), dim3(kThreads), 0, stream, (const int32_t *)x32, k,
+                       24u, s0, nchunks, tiles, slots, nbins);
     CR_HIP(hipGetLastError());
     return CR_OK;
 }

@@ -49,7 +49,7 @@ i128 krawtchouk(uint32_t n, uint32_t d, uint32_t w) {
 
 // Odd part of A, codeword bits n and support of the spectral path at k.
 bool spectral_shape(uint64_t A, uint32_t k, uint64_t *Ao, uint32_t *n) {
-    if (A == 0 || (k != 16 && k != 24 && k != 32)) return false;
+    if (A == 0 || k < 14 || k > 32) return false;
     const uint64_t N = 1ull << k;
     if ((unsigned __int128)(N - 1) * A >> 64) return false;
     uint64_t a = A;
@@ -87,7 +87,7 @@ extern "C" cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t 
     clear_error();
     if (!ctx || !raw) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (A == 0) return fail(CR_ERR_INVALID_ARGUMENT, "A must be >= 1");
-    if (k != 16 && k != 24 && k != 32) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u (16, 24, 32 supported)", k);
+    if (k < 14 || k > 32) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u outside [14, 32]", k);
     const uint64_t N = 1ull << k;
     if ((unsigned __int128)(N - 1) * A >> 64) return fail(CR_ERR_UNSUPPORTED, "codewords (2^k-1)*A exceed 64 bits");
     const uint32_t need = pair_max_distance(A, N) + 1;
@@ -107,10 +107,10 @@ extern "C" cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t 
     // size of the last-level cache (measured at k = 24, A = 58659: 2.36 s with
     // 256 MB, 2.63 s with 128 MB, 2.67 s with 512 MB)
     const uint64_t chunks = 1ull << nhi;
-    const uint64_t chunk_bytes = N * (k == 16 ? sizeof(int16_t) : sizeof(int32_t));
+    const uint64_t chunk_bytes = N * (k <= 16 ? sizeof(int16_t) : sizeof(int32_t));
     const uint64_t B = std::min<uint64_t>(chunks, std::max<uint64_t>(1, (1ull << 28) / chunk_bytes));
     const uint32_t nslots = (uint32_t)ctx->cus * 4;
-    const uint64_t slot_entries = (uint64_t)nslots * 2 * nw;
+    const uint64_t slot_entries = (uint64_t)nslots * 4 * nw;  // per weight: lo sum, carries, hi sum, carries
     CR_TRY(ctx->table_a.reserve(B * chunk_bytes));
     CR_TRY(ctx->hist.reserve(slot_entries * sizeof(uint64_t)));
     CR_TRY(ctx->host.reserve(slot_entries * sizeof(uint64_t)));
@@ -127,12 +127,16 @@ extern "C" cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t 
                           ctx->stream));
     CR_HIP(hipStreamSynchronize(ctx->stream));
 
-    // W(w) = sum over slots of lo + hi * 2^32 (exact in 128 bits: W <= 2^(n+k))
+    // W(w) = sum over slots of lo + hi * 2^32, each a 64-bit sum plus a count
+    // of carries out of it (exact in 128 bits: W <= 2^(n+k) <= 2^80)
     const uint64_t *sl = ctx->host.as<uint64_t>();
     std::vector<u128> W(nw, 0);
     for (uint32_t q = 0; q < nslots; ++q)
-        for (uint32_t w = 0; w < nw; ++w)
-            W[w] += (u128)sl[(uint64_t)q * 2 * nw + 2 * w] + ((u128)sl[(uint64_t)q * 2 * nw + 2 * w + 1] << 32);
+        for (uint32_t w = 0; w < nw; ++w) {
+            const uint64_t *e = sl + (uint64_t)q * 4 * nw + 4 * w;
+            const u128 lo = (u128)e[0] + ((u128)e[1] << 64), hi = (u128)e[2] + ((u128)e[3] << 64);
+            W[w] += lo + (hi << 32);
+        }
     // D(d) = 2^-n sum_w K_d(w) W(w); |terms| < 2^(n+k+45+6) <= 2^123
     memset(raw, 0, nbins * sizeof(uint64_t));
     for (uint32_t d = 0; d < nw; ++d) {

@@ -74,7 +74,7 @@ cr_status cr_device_memory(cr_context *ctx, uint64_t *free_bytes, uint64_t *tota
  * (distance_distribution/src/an_coding.cu:50-105) before run_ancoding doubles it
  * (an_coding.cu:287-292).  With A = A0*s and num_words = m it is the
  * integer-stride grid {A0*(i*s) : i < m}.
- * For num_words = 2^16, 2^24 or 2^32 the same histogram is computed from the code's
+ * For num_words = 2^k, 14 <= k <= 32, the same histogram is computed from the code's
  * Walsh-Hadamard spectrum (cr_an_pairs_spectral) when that is estimated at
  * under 80 % of the pair kernel's time (multipliers well below 2^k);
  * cr_an_pairs_tuned always runs the pair kernel.
@@ -101,9 +101,9 @@ cr_status cr_an_pairs_tuned(cr_context *ctx, uint64_t A, uint64_t num_words, uin
  * A'*(2^k-1), one length-2^k transform per s < 2^(n-k), O(2^n * k) work
  * against the 2^(2k-1) pairs of dancoding_shared
  * (distance_distribution/src/an_coding.cu:50-105), exact integer arithmetic.
- * Supported: k = 16, 24 or 32 and n <= min(2k, 48) (CR_ERR_UNSUPPORTED
- * otherwise); k = 32 gives exact distributions where the pairs (2^63) are out
- * of reach (A = 61: 1.3 s).
+ * Supported: 14 <= k <= 32 and n <= min(2k, 48) (CR_ERR_UNSUPPORTED
+ * otherwise); large k gives exact distributions where the pairs are out of
+ * reach (A = 61: k = 28 in 66 ms, k = 32 in 1.3 s).
  * kernel_ms (may be NULL) receives the device time of all the launches. */
 cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t k, uint64_t *raw, uint32_t nbins,
                                double *kernel_ms);

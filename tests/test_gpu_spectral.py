@@ -31,6 +31,39 @@ def test_k16_recorded_runs(ctx):
         assert report(16, done[r["A"]], len(r["counts"])) == r["counts"], r["src"]
 
 
+@pytest.mark.parametrize("k,nruns", [(14, 64), (15, 64), (18, 15), (20, 4)])
+def test_recorded_runs_other_k(ctx, k, nruns):
+    runs = [r for r in golden("an_exhaustive_runs.json")["runs"] if r["n"] == k]
+    assert len(runs) == nruns
+    done = {}
+    for r in runs:
+        if r["A"] not in done:
+            done[r["A"]], _ = ctx.an_pairs_spectral(r["A"], k)
+        assert report(k, done[r["A"]], len(r["counts"])) == r["counts"], r["src"]
+
+
+@pytest.mark.parametrize("A,k", [(61, 17), (641, 19), (61, 21), (3, 23), (61, 23)])
+def test_narrow_last_pass_matches_pair_kernel(ctx, A, k):
+    raw_s, _ = ctx.an_pairs_spectral(A, k)
+    raw_p, _ = ctx.an_pairs(A, 1 << k)
+    assert raw_s == raw_p
+
+
+@pytest.mark.parametrize("k", [25, 26, 27, 28, 29, 30, 31])
+def test_closed_form_k25_to_31(ctx, k):
+    # A = 1: D(d) = 2^k C(k, d) (the middle pass and the narrow last pass)
+    import math
+    raw, _ = ctx.an_pairs_spectral(1, k)
+    assert raw[1:k + 1] == [(1 << (k - 1)) * math.comb(k, d) for d in range(1, k + 1)]
+
+
+def test_k28_totals(ctx):
+    raw, ms = ctx.an_pairs_spectral(61, 28)
+    print(f"k=28 A=61: {ms:.1f} ms", flush=True)
+    N = 1 << 28
+    assert sum(raw) == N * (N - 1) // 2 and raw[1] == 0
+
+
 def test_k24_recorded_runs(ctx):
     # results/runtimes/*/ancoding_gpu_A*_n24_*.csv and results/As/ancoding_gpu_A*_n24_*.csv:
     # 30- to 40-bit codewords
@@ -134,9 +167,10 @@ def test_k32_exact_against_recorded_grid_approximations(ctx):
 
 def test_unsupported_and_invalid(ctx):
     import coding_reliability_amd as cra
-    with pytest.raises(cra.CRError) as e:
-        ctx.an_pairs_spectral(61, 20)
-    assert e.value.status == 2  # CR_ERR_UNSUPPORTED
+    for k in (13, 33):
+        with pytest.raises(cra.CRError) as e:
+            ctx.an_pairs_spectral(61, k)
+        assert e.value.status == 2  # CR_ERR_UNSUPPORTED
     with pytest.raises(cra.CRError) as e:
         ctx.an_pairs_spectral(1 << 16 | 1, 32)  # 49-bit codewords
     assert e.value.status == 2  # CR_ERR_UNSUPPORTED
