@@ -339,8 +339,9 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
 // owning column t and all 2^R rows in registers (no exchange).  Element
 // (row, column) of tile tau is tau_hi << (tbit + R) | row << tbit |
 // tau_lo << 8 | column, tau_lo being tbit - 8 bits.  F^2 is summed as in
-// spec_last, by the popcount of the row over the base |s| + |tau| + |col|.
-template <class In, int R, bool WIDE>
+// spec_last, by the popcount of the row over the base |s| + |tau| + |col|;
+// k <= 31 here, so F^2 < 2^62.
+template <class In, int R>
 __global__ __launch_bounds__(kThreads) void spec_last_narrow(const In *__restrict__ X, uint32_t k, uint32_t tbit,
                                                              uint64_t s0, uint32_t nchunks,
                                                              uint32_t tiles_per_chunk,
@@ -374,13 +375,9 @@ __global__ __launch_bounds__(kThreads) void spec_last_narrow(const In *__restric
         for (int o_ = 0; o_ <= R; ++o_) lo[o_] = hi[o_] = 0;
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            if (WIDE) {
-                add_square(v[r], lo[__builtin_popcount(r)], hi[__builtin_popcount(r)]);
-            } else {
-                const unsigned long long sq = (unsigned long long)(v[r] * v[r]);  // < 2^63: k <= 31
-                lo[__builtin_popcount(r)] += sq & 0xFFFFFFFFull;
-                hi[__builtin_popcount(r)] += sq >> 32;
-            }
+            const unsigned long long sq = (unsigned long long)(v[r] * v[r]);
+            lo[__builtin_popcount(r)] += sq & 0xFFFFFFFFull;
+            hi[__builtin_popcount(r)] += sq >> 32;
         }
         const uint32_t base = (uint32_t)__popcll(s0 + ch) + __popc(tau) + __popc(t);
 #pragma unroll
@@ -393,13 +390,13 @@ __global__ __launch_bounds__(kThreads) void spec_last_narrow(const In *__restric
     flush_slot(bins, slots + (uint64_t)blockIdx.x * 4 * nbins, nbins);
 }
 
-template <class In, bool WIDE>
+template <class In>
 cr_status launch_narrow(uint32_t R, const In *X, uint32_t k, uint32_t tbit, uint64_t s0, uint32_t nchunks,
                         unsigned long long *slots, uint32_t nslots, uint32_t nbins, hipStream_t stream) {
     const uint32_t tiles = 1u << (k - R - 8);
 #define CR_NARROW(RR)                                                                                            \
     case RR:                                                                                                     \
-        hipLaunchKernelGGL((spec_last_narrow<In, RR, WIDE>), dim3(nslots), dim3(kThreads), 0, stream, X, k, tbit, \
+        hipLaunchKernelGGL((spec_last_narrow<In, RR>), dim3(nslots), dim3(kThreads), 0, stream, X, k, tbit, \
                            s0, nchunks, tiles, slots, nbins);                                                    \
         break;
     switch (R) {
@@ -431,7 +428,7 @@ cr_status launch_spectral_batch(void *X, uint32_t k, uint64_t A, uint32_t Ainv, 
                            Ainv, (uint32_t)tc, (uint32_t)s0);
         CR_HIP(hipGetLastError());
         if (k < 16)
-            return launch_narrow<int16_t, false>(k - 8, x16, k, 8, s0, nchunks, slots, nslots, nbins, stream);
+            return launch_narrow<int16_t>(k - 8, x16, k, 8, s0, nchunks, slots, nslots, nbins, stream);
         hipLaunchKernelGGL((spec_last<int16_t, false>), dim3(nslots), dim3(kThreads), 0, stream,
                            (const int16_t *)x16, k, 8u, s0, nchunks, tiles, slots, nbins);
         CR_HIP(hipGetLastError());
@@ -442,7 +439,7 @@ cr_status launch_spectral_batch(void *X, uint32_t k, uint64_t A, uint32_t Ainv, 
     hipLaunchKernelGGL(spec_gen16, dim3(1u << (k - 16), nchunks), dim3(kThreads16), 0, stream, x32, k, (uint32_t)A,
                        Ainv, (uint32_t)tc, (uint32_t)s0);
     CR_HIP(hipGetLastError());
-    if (k < 24) return launch_narrow<int32_t, false>(k - 16, x32, k, 16, s0, nchunks, slots, nslots, nbins, stream);
+    if (k < 24) return launch_narrow<int32_t>(k - 16, x32, k, 16, s0, nchunks, slots, nslots, nbins, stream);
     if (k == 24) {
         hipLaunchKernelGGL((spec_last<int32_t, false>), dim3(nslots), dim3(kThreads), 0, stream,
                            (const int32_t *)x32, k, 16u, s0, nchunks, tiles, slots, nbins);
@@ -451,7 +448,7 @@ cr_status launch_spectral_batch(void *X, uint32_t k, uint64_t A, uint32_t Ainv, 
     }
     hipLaunchKernelGGL(spec_mid, dim3(tiles, nchunks), dim3(kThreads), 0, stream, x32, k, 16u);
     CR_HIP(hipGetLastError());
-    if (k < 32) return launch_narrow<int32_t, false>(k - 24, x32, k, 24, s0, nchunks, slots, nslots, nbins, stream);
+    if (k < 32) return launch_narrow<int32_t>(k - 24, x32, k, 24, s0, nchunks, slots, nslots, nbins, stream);
     hipLaunchKernelGGL((spec_last<int32_t, true>), dim3(nslots), dim3(kThreads), 0, stream, (const int32_t *)x32, k,
                        24u, s0, nchunks, tiles, slots, nbins);
     CR_HIP(hipGetLastError());
