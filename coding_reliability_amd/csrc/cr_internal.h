@@ -157,6 +157,21 @@ constexpr uint32_t kHamMaxLaneSamples = 4096;
 // 9.6e12 pairs/s).
 bool spectral_preferred(uint64_t A, uint64_t num_words, uint32_t *k);
 
+// One exhaustive AN histogram by the code's spectrum, split into batches of
+// transforms (see spectral.hip): shape, device buffer and slot sizes.
+struct SpectralPlan {
+    uint64_t Ao = 0, tc = 0, chunks = 0, per_batch = 0, batches = 0, buffer_bytes = 0, slot_entries = 0;
+    uint32_t k = 0, n = 0, nw = 0, Ainv = 0, nslots = 0;
+};
+// False when (A, k) is outside the spectral path (then the pair kernel applies).
+bool spectral_plan(uint64_t A, uint32_t k, int cus, SpectralPlan *p);
+// Launches batches [first_batch, first_batch + nbatches) on stream, with X of
+// p.buffer_bytes and slots of p.slot_entries uint64 (zeroed before the first).
+cr_status spectral_run(const SpectralPlan &p, void *X, unsigned long long *slots, uint64_t first_batch,
+                       uint64_t nbatches, hipStream_t stream);
+// raw[] (nbins entries, pair-kernel form) from the host copy of the slots.
+cr_status spectral_finish(const SpectralPlan &p, const uint64_t *slots, uint64_t *raw, uint32_t nbins);
+
 // Parity masks of the (N,k) extended Hamming codes, truncated to k bits;
 // returns the number of non-overall parity bits.
 uint32_t hamming_masks(uint32_t k, uint64_t masks[7]);

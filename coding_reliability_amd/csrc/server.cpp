@@ -58,6 +58,11 @@ struct Request {
     uint64_t units = 0;
     PairJob pj{};
     HamJob hj{};
+    // exhaustive requests cheaper through the code's spectrum: items are
+    // batches of transforms, run after the pair and Hamming launches
+    bool spectral = false;
+    SpectralPlan sp{};
+    DeviceBuffer sx, sslots;
     uint32_t iterations = 0;
     double t_submit = 0, t_admit = 0, t_finish = 0;
     std::vector<uint64_t> result;
@@ -69,7 +74,7 @@ struct cr_server {
     cr_context *ctx = nullptr;
     cr_server_config cfg{};
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr, ev2 = nullptr;
     DeviceBuffer slab, pair_jobs, ham_jobs, counter, slot_list;
     HostBuffer staging, readback;
     std::vector<int> free_slots;
@@ -131,6 +136,14 @@ cr_status describe(cr_server *s, const cr_job &j, Request &r) {
         r.nbins = p.nbins;
         r.total_items = p.items;
         r.units = N * (N - 1) / 2;
+        // the spectral path when it is cheaper and its buffers fit the budget
+        // (otherwise the pair kernel, whose footprint is one histogram slot)
+        uint32_t k = 0;
+        if (j.kind == CR_JOB_AN_EXHAUSTIVE && spectral_preferred(A, N, &k) && spectral_plan(A, k, s->ctx->cus, &r.sp) &&
+            r.sp.buffer_bytes + r.sp.slot_entries * sizeof(uint64_t) + 4096 <= s->cfg.hbm_budget_bytes) {
+            r.spectral = true;
+            r.total_items = r.sp.batches;
+        }
     } else if (j.kind == CR_JOB_HAMMING_MC) {
         if (j.k < 1 || j.k > 64) return fail(CR_ERR_INVALID_ARGUMENT, "Hamming request k=%u outside [1, 64]", j.k);
         HamJob &h = r.hj;
@@ -152,6 +165,7 @@ cr_status describe(cr_server *s, const cr_job &j, Request &r) {
     }
     r.units_per_item = r.total_items ? (long double)r.units / r.total_items : 1;
     r.footprint = ((uint64_t)kSlotBins * sizeof(uint64_t) + 255) / 256 * 256;
+    if (r.spectral) r.footprint += r.sp.buffer_bytes + r.sp.slot_entries * sizeof(uint64_t);
     return CR_OK;
 }
 
@@ -171,16 +185,26 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
         if (r->kind == CR_JOB_HAMMING_MC) {
             lds_ham = std::max(lds_ham, r->hj.nbins + 1);
             ++n_mc;
-        } else {
+        } else if (!r->spectral) {
             lds_pair = std::max(lds_pair, r->pj.nbins + 1);
             ++n_an;
         }
     }
+    std::vector<std::pair<Request *, uint64_t>> spec;
     // waves each launch runs with: a slice floor of two items per wave
     const uint64_t wp = n_an ? (uint64_t)pair_waves_per_cu(cfg.rows_per_lane, lds_pair) * ctx->cus : 0;
     const uint64_t wh = n_mc ? (uint64_t)hamming_waves_per_cu(lds_ham) * ctx->cus : 0;
     long double pair_units = 0, ham_units = 0;
     for (Request *r : run) {
+        if (r->spectral) {
+            // batches for the share at 2.2e-9 ms per transformed element
+            const long double batch_ms = (long double)r->sp.per_batch * (long double)(1ull << r->sp.k) * 2.2e-9L;
+            uint64_t n = std::max<uint64_t>((uint64_t)(share / batch_ms), 1);
+            n = std::min<uint64_t>(n, r->total_items - r->cursor);
+            spec.push_back({r, n});
+            took.push_back({r, n});
+            continue;
+        }
         const bool mc = r->kind == CR_JOB_HAMMING_MC;
         const long double rate = mc ? sample_rate : pair_rate;
         uint64_t n = (uint64_t)(share * rate / std::max<long double>(r->units_per_item, 1));
@@ -267,9 +291,19 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
         CR_TRY(launch_hamming(ham_jobs.as<HamJob>() + h32.size(), (uint32_t)h64.size(), t64, true, ctr + 2, false,
                               lds_ham, ctx->cus, stream, nullptr, nullptr));
     CR_HIP(hipEventRecord(ev1, stream));
+    for (auto &sr : spec) {
+        Request *r = sr.first;
+        if (!r->sx.ptr) {  // first slice: transform buffer and zeroed slots
+            CR_TRY(r->sx.reserve(r->sp.buffer_bytes));
+            CR_TRY(r->sslots.reserve(r->sp.slot_entries * sizeof(uint64_t)));
+            CR_TRY(fill_zero_u64(r->sslots.as<unsigned long long>(), r->sp.slot_entries, stream));
+        }
+        CR_TRY(spectral_run(r->sp, r->sx.ptr, r->sslots.as<unsigned long long>(), r->cursor, sr.second, stream));
+    }
+    CR_HIP(hipEventRecord(ev2, stream));
     CR_HIP(hipStreamSynchronize(stream));
     float ms = 0, pms = 0, hms = 0;
-    CR_HIP(hipEventElapsedTime(&ms, ev0, ev1));
+    CR_HIP(hipEventElapsedTime(&ms, ev0, ev2));
     CR_HIP(hipEventElapsedTime(&pms, ev0, evm));
     CR_HIP(hipEventElapsedTime(&hms, evm, ev1));
     // refine the throughput model (exponential average) from launches that
@@ -301,6 +335,8 @@ void cr_server::loop() {
                     r->state = 2;
                     r->status = CR_ERR_CANCELLED;
                     r->t_finish = now();
+                    r->sx.release();
+                    r->sslots.release();
                     footprint -= r->footprint;
                     free_slots.push_back(r->slot);
                     dirty_slots.push_back((uint32_t)r->slot);
@@ -370,12 +406,22 @@ void cr_server::loop() {
         }
         for (Request *r : fin) {
             std::vector<uint64_t> h(r->nbins, 0);
-            if (s == CR_OK)
+            cr_status rs = s;
+            if (rs == CR_OK && r->spectral) {
+                // the histogram from the request's slots (Krawtchouk transform on the host)
+                std::vector<uint64_t> sl(r->sp.slot_entries);
+                hipError_t e = hipMemcpy(sl.data(), r->sslots.ptr, sl.size() * sizeof(uint64_t), hipMemcpyDeviceToHost);
+                rs = e == hipSuccess ? spectral_finish(r->sp, sl.data(), h.data(), r->nbins)
+                                     : fail(CR_ERR_DEVICE, "spectral result copy failed: %s", hipGetErrorString(e));
+            } else if (rs == CR_OK) {
                 memcpy(h.data(), readback.as<uint64_t>() + (size_t)(r->slot - lo) * kSlotBins,
                        r->nbins * sizeof(uint64_t));
+            }
+            r->sx.release();
+            r->sslots.release();
             std::lock_guard<std::mutex> lk(mu);
             r->result.swap(h);
-            r->status = s;
+            r->status = rs;
             r->state = 2;
             r->t_finish = now();
             footprint -= r->footprint;
@@ -410,6 +456,7 @@ cr_status cr_server_create(cr_context *ctx, const cr_server_config *cfg, cr_serv
         CR_HIP(hipEventCreate(&raw->ev0));
         CR_HIP(hipEventCreate(&raw->evm));
         CR_HIP(hipEventCreate(&raw->ev1));
+        CR_HIP(hipEventCreate(&raw->ev2));
         const size_t slots = (size_t)raw->cfg.max_active * kSlotBins;
         CR_TRY(raw->slab.reserve(slots * sizeof(uint64_t)));
         CR_TRY(fill_zero_u64(raw->slab.as<unsigned long long>(), slots, raw->stream));
@@ -532,6 +579,10 @@ cr_status cr_server_destroy(cr_server *s) {
     if (s->worker.joinable()) s->worker.join();
     (void)hipSetDevice(s->ctx->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (auto &kv : s->reqs) {
+        kv.second->sx.release();
+        kv.second->sslots.release();
+    }
     s->slab.release();
     s->pair_jobs.release();
     s->ham_jobs.release();
@@ -542,6 +593,7 @@ cr_status cr_server_destroy(cr_server *s) {
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->evm) (void)hipEventDestroy(s->evm);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->ev2) (void)hipEventDestroy(s->ev2);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     delete s;
     return CR_OK;

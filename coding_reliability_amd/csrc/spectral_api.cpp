@@ -78,6 +78,68 @@ bool spectral_preferred(uint64_t A, uint64_t num_words, uint32_t *k) {
     return spectral_s < 0.8 * pair_s;
 }
 
+bool spectral_plan(uint64_t A, uint32_t k, int cus, SpectralPlan *p) {
+    uint64_t Ao = 0;
+    uint32_t n = 0;
+    if (!spectral_shape(A, k, &Ao, &n)) return false;
+    const uint64_t N = 1ull << k;
+    p->Ao = Ao;
+    p->k = k;
+    p->n = n;
+    p->nw = n + 1;  // weights 0..n
+    p->Ainv = inverse_mod_2_32((uint32_t)Ao) & (uint32_t)(N - 1);
+    p->tc = (Ao * p->Ainv - 1) >> k;  // Ao * Ainv = 1 + tc * 2^k (below 2^56)
+    p->chunks = 1ull << (n - k);
+    // chunks of 2^k per launch: 256 MB of transforms in flight, about the
+    // size of the last-level cache (measured at k = 24, A = 58659: 2.36 s with
+    // 256 MB, 2.63 s with 128 MB, 2.67 s with 512 MB)
+    const uint64_t chunk_bytes = N * (k <= 16 ? sizeof(int16_t) : sizeof(int32_t));
+    p->per_batch = std::min<uint64_t>(p->chunks, std::max<uint64_t>(1, (1ull << 28) / chunk_bytes));
+    p->batches = (p->chunks + p->per_batch - 1) / p->per_batch;
+    p->buffer_bytes = p->per_batch * chunk_bytes;
+    p->nslots = (uint32_t)cus * 4;
+    p->slot_entries = (uint64_t)p->nslots * 4 * p->nw;  // per weight: lo sum, carries, hi sum, carries
+    return true;
+}
+
+cr_status spectral_run(const SpectralPlan &p, void *X, unsigned long long *slots, uint64_t first_batch,
+                       uint64_t nbatches, hipStream_t stream) {
+    for (uint64_t b = first_batch; b < first_batch + nbatches && b < p.batches; ++b) {
+        const uint64_t s0 = b * p.per_batch;
+        const uint32_t nb = (uint32_t)std::min<uint64_t>(p.per_batch, p.chunks - s0);
+        CR_TRY(launch_spectral_batch(X, p.k, p.Ao, p.Ainv, p.tc, s0, nb, slots, p.nslots, p.nw, stream));
+    }
+    return CR_OK;
+}
+
+cr_status spectral_finish(const SpectralPlan &p, const uint64_t *sl, uint64_t *raw, uint32_t nbins) {
+    // W(w) = sum over slots of lo + hi * 2^32, each a 64-bit sum plus a count
+    // of carries out of it (exact in 128 bits: W <= 2^(n+k) <= 2^80)
+    std::vector<u128> W(p.nw, 0);
+    for (uint32_t q = 0; q < p.nslots; ++q)
+        for (uint32_t w = 0; w < p.nw; ++w) {
+            const uint64_t *e = sl + (uint64_t)q * 4 * p.nw + 4 * w;
+            const u128 lo = (u128)e[0] + ((u128)e[1] << 64), hi = (u128)e[2] + ((u128)e[3] << 64);
+            W[w] += lo + (hi << 32);
+        }
+    // D(d) = 2^-n sum_w K_d(w) W(w); partial sums stay below C(n, d) 2^(n+k) <= 2^125
+    memset(raw, 0, nbins * sizeof(uint64_t));
+    for (uint32_t d = 0; d < p.nw; ++d) {
+        i128 s = 0;
+        for (uint32_t w = 0; w < p.nw; ++w) s += krawtchouk(p.n, d, w) * (i128)W[w];
+        const i128 unit = (i128)1 << p.n;
+        if (s < 0 || s % unit != 0) return fail(CR_ERR_DEVICE, "spectral sums inconsistent at distance %u", d);
+        const i128 D = s / unit;
+        if (d == 0) {
+            if (D != ((i128)1 << p.k)) return fail(CR_ERR_DEVICE, "spectral sums inconsistent at distance 0");
+            continue;
+        }
+        if (D & 1) return fail(CR_ERR_DEVICE, "odd ordered-pair count at distance %u", d);
+        raw[d] = (uint64_t)(D / 2);
+    }
+    return CR_OK;
+}
+
 }  // namespace cr
 
 using namespace cr;
@@ -92,66 +154,23 @@ extern "C" cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t 
     if ((unsigned __int128)(N - 1) * A >> 64) return fail(CR_ERR_UNSUPPORTED, "codewords (2^k-1)*A exceed 64 bits");
     const uint32_t need = pair_max_distance(A, N) + 1;
     if (nbins < need) return fail(CR_ERR_INVALID_ARGUMENT, "nbins=%u but distances reach %u", nbins, need - 1);
-    uint64_t Ao = 0;
-    uint32_t n = 0;
-    if (!spectral_shape(A, k, &Ao, &n))
+    SpectralPlan p;
+    if (!spectral_plan(A, k, ctx->cus, &p))
         return fail(CR_ERR_UNSUPPORTED, "spectral path: A=%llu at k=%u (odd part below 2^32, codewords of at most "
                     "min(2k, 48) bits)", (unsigned long long)A, k);
-    const uint32_t nhi = n - k, nw = n + 1;  // weights 0..n
-    const uint32_t mask = (uint32_t)(N - 1);
-    const uint32_t Ainv = inverse_mod_2_32((uint32_t)Ao) & mask;
-    const uint64_t tc = (Ao * Ainv - 1) >> k;  // Ao * Ainv = 1 + tc * 2^k (below 2^56)
     CR_HIP(hipSetDevice(ctx->device));
-
-    // chunks of 2^k per launch: 256 MB of transforms in flight, about the
-    // size of the last-level cache (measured at k = 24, A = 58659: 2.36 s with
-    // 256 MB, 2.63 s with 128 MB, 2.67 s with 512 MB)
-    const uint64_t chunks = 1ull << nhi;
-    const uint64_t chunk_bytes = N * (k <= 16 ? sizeof(int16_t) : sizeof(int32_t));
-    const uint64_t B = std::min<uint64_t>(chunks, std::max<uint64_t>(1, (1ull << 28) / chunk_bytes));
-    const uint32_t nslots = (uint32_t)ctx->cus * 4;
-    const uint64_t slot_entries = (uint64_t)nslots * 4 * nw;  // per weight: lo sum, carries, hi sum, carries
-    CR_TRY(ctx->table_a.reserve(B * chunk_bytes));
-    CR_TRY(ctx->hist.reserve(slot_entries * sizeof(uint64_t)));
-    CR_TRY(ctx->host.reserve(slot_entries * sizeof(uint64_t)));
+    CR_TRY(ctx->table_a.reserve(p.buffer_bytes));
+    CR_TRY(ctx->hist.reserve(p.slot_entries * sizeof(uint64_t)));
+    CR_TRY(ctx->host.reserve(p.slot_entries * sizeof(uint64_t)));
     unsigned long long *slots = ctx->hist.as<unsigned long long>();
-    CR_TRY(fill_zero_u64(slots, slot_entries, ctx->stream));
+    CR_TRY(fill_zero_u64(slots, p.slot_entries, ctx->stream));
     if (kernel_ms) CR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    for (uint64_t s0 = 0; s0 < chunks; s0 += B) {
-        const uint32_t nb = (uint32_t)std::min<uint64_t>(B, chunks - s0);
-        CR_TRY(launch_spectral_batch(ctx->table_a.ptr, k, Ao, Ainv, tc, s0, nb, slots, nslots, nw,
-                                     ctx->stream));
-    }
+    CR_TRY(spectral_run(p, ctx->table_a.ptr, slots, 0, p.batches, ctx->stream));
     if (kernel_ms) CR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-    CR_HIP(hipMemcpyAsync(ctx->host.ptr, slots, slot_entries * sizeof(uint64_t), hipMemcpyDeviceToHost,
+    CR_HIP(hipMemcpyAsync(ctx->host.ptr, slots, p.slot_entries * sizeof(uint64_t), hipMemcpyDeviceToHost,
                           ctx->stream));
     CR_HIP(hipStreamSynchronize(ctx->stream));
-
-    // W(w) = sum over slots of lo + hi * 2^32, each a 64-bit sum plus a count
-    // of carries out of it (exact in 128 bits: W <= 2^(n+k) <= 2^80)
-    const uint64_t *sl = ctx->host.as<uint64_t>();
-    std::vector<u128> W(nw, 0);
-    for (uint32_t q = 0; q < nslots; ++q)
-        for (uint32_t w = 0; w < nw; ++w) {
-            const uint64_t *e = sl + (uint64_t)q * 4 * nw + 4 * w;
-            const u128 lo = (u128)e[0] + ((u128)e[1] << 64), hi = (u128)e[2] + ((u128)e[3] << 64);
-            W[w] += lo + (hi << 32);
-        }
-    // D(d) = 2^-n sum_w K_d(w) W(w); |terms| < 2^(n+k+45+6) <= 2^123
-    memset(raw, 0, nbins * sizeof(uint64_t));
-    for (uint32_t d = 0; d < nw; ++d) {
-        i128 s = 0;
-        for (uint32_t w = 0; w < nw; ++w) s += krawtchouk(n, d, w) * (i128)W[w];
-        const i128 unit = (i128)1 << n;
-        if (s < 0 || s % unit != 0) return fail(CR_ERR_DEVICE, "spectral sums inconsistent at distance %u", d);
-        const i128 D = s / unit;
-        if (d == 0) {
-            if (D != (i128)N) return fail(CR_ERR_DEVICE, "spectral sums inconsistent at distance 0");
-            continue;
-        }
-        if (D & 1) return fail(CR_ERR_DEVICE, "odd ordered-pair count at distance %u", d);
-        raw[d] = (uint64_t)(D / 2);
-    }
+    CR_TRY(spectral_finish(p, ctx->host.as<uint64_t>(), raw, nbins));
     if (kernel_ms) {
         float ms = 0;
         CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));

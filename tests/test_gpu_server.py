@@ -6,6 +6,7 @@ import time
 import pytest
 
 import coding_reliability_amd as cra
+from conftest import padded_eq
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -38,6 +39,32 @@ def direct(ctx, j):
         return ctx.an_pairs(j["A"] * s, ((1 << j["k"]) + s - 1) // s)[0]
     return ctx.hamming_weights(j["k"], cra.HAM_MC, first=0, count=j["samples"], seed=j["seed"],
                                stream=j["stream"])[0]
+
+
+def test_spectral_requests_in_a_mix(ctx, solutions):
+    # exhaustive requests the spectral path is cheaper for (A = 61 at k = 16
+    # and 24, A = 58659 at k = 24: 2.4 s instead of 15 s) are served as
+    # batches of transforms beside pair-kernel and Monte Carlo requests
+    jobs = [dict(kind=cra.JOB_AN_EXHAUSTIVE, k=24, A=58659),
+            dict(kind=cra.JOB_AN_EXHAUSTIVE, k=24, A=61),
+            dict(kind=cra.JOB_AN_EXHAUSTIVE, k=18, A=58659),
+            dict(kind=cra.JOB_AN_EXHAUSTIVE, k=16, A=61),
+            dict(kind=cra.JOB_HAMMING_MC, k=32, samples=1 << 22, seed=3, stream=4)]
+    t0 = time.perf_counter()
+    with cra.Server(ctx, iteration_ms=5) as srv:
+        ids = [srv.submit(**j) for j in jobs]
+        for rid in ids:
+            srv.wait(rid)
+        got = [srv.result(rid)[0] for rid in ids]
+    dt = time.perf_counter() - t0
+    print(f"spectral mix served in {dt:.2f} s", flush=True)
+    assert dt < 10  # the k = 24, A = 58659 request alone takes 15 s on the pair kernel
+    assert got[0] == ctx.an_pairs_spectral(58659, 24)[0][: len(got[0])]
+    assert padded_eq(O.an_report(24, got[1], 31), solutions["solution_an24_A61"])
+    for j, g in zip(jobs[2:], got[2:]):
+        assert g == direct(ctx, j)[: len(g)], j
+    N = 1 << 24
+    assert sum(got[0]) == N * (N - 1) // 2
 
 
 def test_alone_equals_mix(ctx):
