@@ -32,6 +32,7 @@ K_HEADLINE = 24
 PEAK_CLOCK_GHZ = 2.4
 LANES_PER_CU_CLK = 64
 LDS_ATOMIC_LANES_PER_CU_CLK = 16
+HBM_PEAK_TBS = 8.0  # MI355X HBM3E, published
 
 
 def log(msg: str) -> None:
@@ -259,6 +260,17 @@ def run_spectral(ctx, A: int, k: int, raw_pairs: list, pair_kernel_s: float) -> 
     N = 1 << k
     Ao = A // (A & -A)  # odd part: the same distances
     n = max((Ao * (N - 1)).bit_length(), k)
+    # memory roofline of the passes: each transformed element is written once
+    # as int32 by the first pass and read once by the last (8 bytes; k = 24)
+    elements = 1 << n
+    bytes_per_element = 8 if 17 <= k <= 24 else None
+    memory = None
+    if bytes_per_element:
+        achieved = elements * bytes_per_element / (ms / 1e3) / 1e12
+        memory = {"bytes_per_element": bytes_per_element, "achieved": round(achieved, 3), "peak": HBM_PEAK_TBS,
+                  "unit": "TB/s", "frac": round(achieved / HBM_PEAK_TBS, 4),
+                  "note": "the first pass is bound by its instruction work (the +-1 generation and 16 butterfly "
+                          "stages), the last by memory"}
     return {
         "method": "Walsh-Hadamard spectrum of the code: 2^(n-k) transforms of length 2^k, exact integer sums, "
                   "Krawtchouk transform on the host",
@@ -268,6 +280,7 @@ def run_spectral(ctx, A: int, k: int, raw_pairs: list, pair_kernel_s: float) -> 
         "speedup_vs_pair_kernel": round(pair_kernel_s / dt, 2),
         "transforms": 1 << (n - k),
         "transform_length": N,
+        "memory_roofline": memory,
         "bit_exact_vs_pair_kernel": True,
     }
 
