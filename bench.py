@@ -110,7 +110,7 @@ def run_batched(ctx, autotune: bool = True) -> dict:
     t0 = time.perf_counter()
     per = [ctx.an_pairs(A, N)[0] for A, N in codes]
     t_per = time.perf_counter() - t0
-    log(f"C4 per-launch: {t_per:.3f} s")
+    log(f"4096 codes, one launch per code: {t_per:.3f} s")
     t0 = time.perf_counter()
     got, kms = ctx.an_batch(codes, **tune)
     t_batch = time.perf_counter() - t0
@@ -125,8 +125,8 @@ def run_batched(ctx, autotune: bool = True) -> dict:
     t_graph /= reps
     plan.close()
     assert got == per, "batched launch differs from per-code launches"
-    log(f"C4 batched: {t_batch:.3f} s, graph replay: {t_graph:.3f} s")
-    out["c4"] = {
+    log(f"4096 codes batched: {t_batch:.3f} s, graph replay: {t_graph:.3f} s")
+    out["codes4096"] = {
         "workload": "4096 AN codes, odd A uniform in [3,2^16), k uniform in {8..18}, seed 20261202",
         "pairs": pairs,
         "per_launch_s": round(t_per, 4), "per_launch_gpairs": round(pairs / t_per / 1e9, 2),

@@ -56,7 +56,7 @@ def test_c4_full_batch_matches_single_launches(ctx):
     4096 one-code launches bit for bit; k <= 12 codes also checked on the CPU."""
     codes = workloads.c4_codes()
     got, ms = ctx.an_batch(codes)
-    print(f"C4 batch: {ms:.1f} ms, {workloads.pairs_of(codes) / ms / 1e9:.2f} Tpairs/s", flush=True)
+    print(f"4096-code batch: {ms:.1f} ms, {workloads.pairs_of(codes) / ms / 1e9:.2f} Tpairs/s", flush=True)
     for i, ((A, N), h) in enumerate(zip(codes, got)):
         single, _ = ctx.an_pairs(A, N)
         assert h == single, (i, A, N)

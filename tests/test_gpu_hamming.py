@@ -101,7 +101,7 @@ def test_mc_bitexact_vs_oracle(ctx, k, first, count):
 
 
 def test_mc_c3_2pow34_within_4sigma(ctx):
-    # C3 Monte Carlo: 2^34 Philox words of the (39,32) code vs the exhaustive truth
+    # Monte Carlo over 2^34 Philox words of the (39,32) code vs the exhaustive truth
     k, S = 32, 1 << 34
     h, ms = ctx.hamming_weights(k, cra.HAM_MC, first=0, count=S, seed=20261202, stream=0)
     truth, _ = ctx.hamming_weights(k)
