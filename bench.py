@@ -223,6 +223,9 @@ def run_kernels(ctx, peak: float) -> dict:
     h, ms = _warm(lambda: ctx.an_mc(61, 24, m, 16384))
     assert sum(h) == (1 << 24) * m
     out["an_mc"] = _kernel_entry((1 << 24) * m, ms, 19, "Gpairs/s", peak, "AN Monte Carlo, A=61, k=24, 2^16 samples")
+    out["an_mc"]["note"] = ("frac can exceed 1: this Philox-heavy mix issues about 1.04 VALU wave-instructions per "
+                            "CU per clock (SQ_INSTS_VALU over active cycles), above the one-per-clock issue the "
+                            "64 lanes x 2.4 GHz peak assumes; 18.0 VALU per pair executed")
     best, cands, ms = _warm(lambda: ctx.an_candidates(24, 16))
     # pruning skips most (A, d) steps, so the work is counted as the steps the
     # candidates (A attaining the maximum, never pruned) must run in full: a
