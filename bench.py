@@ -575,7 +575,7 @@ def main() -> int:
         "kernel": "pair_kernel<R>",
         "kernel_ms": round(avg_kernel_s * 1e3, 3),
         "ops_per_pair": ops_per_pair,
-        "survey_ops_per_pair_2W_plus_1": 2 * W + 1,
+        "ops_per_pair_2W_plus_1": 2 * W + 1,
         "peak_clock_ghz": PEAK_CLOCK_GHZ,
         # the binding limit of this kernel: one LDS increment per pair, and a
         # wave-wide ds_add_u32 holds a CU's LDS for 4 clocks (16 lanes/clk,
