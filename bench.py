@@ -12,7 +12,7 @@ Prints ONE JSON line (rank 0).  Extra keys beyond the required ones:
   roofline       VALU roofline of the pair kernel (events measured live)
   cpu_baseline   the oracle's restatement of the reference CPU loop on a
                  bounded row-shard sample of the same workload, host cores
-  batched        batched (C4, one launch + captured hipGraph) vs one code per
+  batched        4096 mixed-size codes batched (one launch + captured hipGraph) vs one code per
                  launch, and the super-A k=8 sweep (skip with --no-batched)
   served         continuous-batching request mix (with --served)
 """
@@ -93,14 +93,14 @@ def cpu_baseline(A: int, k: int, target_s: float) -> dict:
 
 
 def run_batched(ctx, autotune: bool = True) -> dict:
-    """C4: 4096 AN codes (odd A uniform in [3, 2^16), k uniform in 8..18, seed
-    C4_SEED) -- one code per launch (cr_an_pairs per code: zero, kernel,
+    """4096 AN codes (odd A uniform in [3, 2^16), k uniform in 8..18, seed
+    MIXED_CODES_SEED) -- one code per launch (cr_an_pairs per code: zero, kernel,
     copy-back, sync) against one batched launch and its captured hipGraph;
     plus the reference's published per-launch sweep shape (super-A, k=8,
     16384 codes).  Every result is checked bit-exact against the per-code run."""
     from coding_reliability_amd import workloads as WL
     out = {}
-    codes = WL.c4_codes()
+    codes = WL.mixed_codes()
     pairs = WL.pairs_of(codes)
     tune = {}
     if autotune:
@@ -351,7 +351,7 @@ def run_vs_reference(ctx) -> dict:
 
 
 def run_served(ctx, njobs: int, rate: float, seed: int = 7, max_wait_s: float = 300.0) -> dict:
-    """C5: Poisson arrivals (rate jobs/s) of a uniform mix of exhaustive AN
+    """Served mix: Poisson arrivals (rate jobs/s) of a uniform mix of exhaustive AN
     (k in 18..24, odd 16-bit A), integer-grid AN (k=32, stride 4097, odd 16-bit
     A) and Monte Carlo (72,64) Extended Hamming (2^30 samples) requests."""
     import numpy as np

@@ -7,10 +7,10 @@ from __future__ import annotations
 
 import numpy as np
 
-C4_SEED = 20261202
+MIXED_CODES_SEED = 20261202
 
 
-def c4_codes(ncodes: int = 4096, seed: int = C4_SEED, kmin: int = 8, kmax: int = 18) -> list[tuple[int, int]]:
+def mixed_codes(ncodes: int = 4096, seed: int = MIXED_CODES_SEED, kmin: int = 8, kmax: int = 18) -> list[tuple[int, int]]:
     """Batched configuration: odd A uniform in [3, 2^16), k uniform in {kmin..kmax};
     returns (A, 2^k) pairs."""
     g = np.random.Generator(np.random.Philox(seed))

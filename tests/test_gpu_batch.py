@@ -54,7 +54,7 @@ def test_plan_graph_replays(ctx):
 def test_c4_full_batch_matches_single_launches(ctx):
     """The batched configuration (4096 codes, k in 8..18): one launch equals
     4096 one-code launches bit for bit; k <= 12 codes also checked on the CPU."""
-    codes = workloads.c4_codes()
+    codes = workloads.mixed_codes()
     got, ms = ctx.an_batch(codes)
     print(f"4096-code batch: {ms:.1f} ms, {workloads.pairs_of(codes) / ms / 1e9:.2f} Tpairs/s", flush=True)
     for i, ((A, N), h) in enumerate(zip(codes, got)):
@@ -67,7 +67,7 @@ def test_c4_full_batch_matches_single_launches(ctx):
 
 
 def test_autotune_small(ctx):
-    codes = workloads.c4_codes(64, seed=3, kmax=14)
+    codes = workloads.mixed_codes(64, seed=3, kmax=14)
     best, ms = ctx.autotune_batch(codes, repeats=1)
     assert best["rows_per_lane"] in (4, 8, 16) and best["items_per_wave"] in (4, 16, 64) and ms > 0
     got, _ = ctx.an_batch(codes, **best)
