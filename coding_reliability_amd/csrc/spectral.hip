@@ -252,15 +252,14 @@ __device__ __forceinline__ void flush_slot(const unsigned long long *bins, unsig
 }
 
 // F^2 into the 32-bit halves lo, hi (|f| <= 2^32: F^2 = 2^64 is hi = 2^32).
+// Branch free: |f| = t * 2^32 + a with t in {0, 1} and a = 0 when t = 1, so
+// F^2 = a^2 + t * 2^64, one 32 x 32 -> 64-bit product.
 __device__ __forceinline__ void add_square(int64_t f, unsigned long long &lo, unsigned long long &hi) {
-    const unsigned long long a = (unsigned long long)(f < 0 ? -f : f);
-    if (a >> 32) {
-        hi += 1ull << 32;
-    } else {
-        const unsigned long long q = a * a;
-        lo += q & 0xFFFFFFFFull;
-        hi += q >> 32;
-    }
+    const unsigned long long m = (unsigned long long)(f < 0 ? -f : f);
+    const uint32_t a = (uint32_t)m, t = (uint32_t)(m >> 32);
+    const unsigned long long q = (unsigned long long)a * a;
+    lo += q & 0xFFFFFFFFull;
+    hi += ((unsigned long long)t << 32) | (q >> 32);
 }
 
 // Last pass: bits [tbit, tbit + 8) of every chunk of the batch (tiles
