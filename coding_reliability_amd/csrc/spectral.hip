@@ -289,8 +289,18 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
         unsigned long long lo[7] = {0, 0, 0, 0, 0, 0, 0}, hi[7] = {0, 0, 0, 0, 0, 0, 0};
         int32_t w[64];
         if (WIDE) {
+            // k = 32: rows lie 2^26 bytes apart, so the wave's 64 rows span
+            // under 2^32 bytes from its first; one buffer resource per wave
+            // (p is the wave index), the row offset a scalar, the column one
+            // 32-bit offset: no 64-bit address per load to hold in VGPRs
+            const uint32_t pw = __builtin_amdgcn_readfirstlane(p);
+            const In *row0 = src + pass_index(tau, tbit, pw * 64, 0);
+            const uint32_t row_bytes = (uint32_t)sizeof(In) << tbit;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<In *>(row0), 0, (int)(63u * row_bytes + 64u * sizeof(In)), 0x00020000);
 #pragma unroll
-            for (int j = 0; j < 64; ++j) w[j] = (int32_t)src[pass_index(tau, tbit, p * 64 + j, c)];
+            for (int j = 0; j < 64; ++j)
+                w[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(c * sizeof(In)), (int)(j * row_bytes), 0);
             wht64(w);  // |w| <= 2^30
             int32_t o[64];
             exchange_gather(w, o, E, c, p);
