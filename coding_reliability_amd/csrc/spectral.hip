@@ -215,12 +215,33 @@ __device__ __forceinline__ uint64_t pass_index(uint32_t tau, uint32_t tbit, uint
 
 // Loads the 256 x 64 tile (rows strided by 2^tbit, 64 contiguous columns)
 // and transforms its rows; w then holds rows held_row(i, p).
+// Rows p * 64 + j (j < 64) of tile tau, column c, into w[j].  For int32 data
+// one buffer resource per wave (p is the wave index) covers the wave's 64
+// rows, under 2^32 bytes for tbit <= 24: the row offset is a scalar and the
+// column one 32-bit VGPR, so no 64-bit address per load occupies VGPRs.
+template <class In>
+__device__ __forceinline__ void load_rows(const In *chunk, uint32_t tau, uint32_t tbit, uint32_t c, uint32_t p,
+                                          int32_t (&w)[64]) {
+    if constexpr (sizeof(In) == 4) {
+        const uint32_t pw = __builtin_amdgcn_readfirstlane(p);
+        const In *row0 = chunk + pass_index(tau, tbit, pw * 64, 0);
+        const uint32_t row_bytes = 4u << tbit;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<In *>(row0), 0, (int)(63u * row_bytes + 64u * 4u), 0x00020000);
+#pragma unroll
+        for (int j = 0; j < 64; ++j)
+            w[j] = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)(c * 4u), (int)(j * row_bytes), 0);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 64; ++j) w[j] = (int32_t)chunk[pass_index(tau, tbit, p * 64 + j, c)];
+    }
+}
+
 template <class In>
 __device__ __forceinline__ void pass_rows(const In *chunk, uint32_t tau, uint32_t tbit, int32_t *E,
                                           int32_t (&w)[64]) {
     const uint32_t t = threadIdx.x, c = t & 63, p = t >> 6;
-#pragma unroll
-    for (int j = 0; j < 64; ++j) w[j] = (int32_t)chunk[pass_index(tau, tbit, p * 64 + j, c)];
+    load_rows<In>(chunk, tau, tbit, c, p, w);
     wht64(w);
     exchange_m(w, E, c, p);
 }
@@ -289,19 +310,8 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
         unsigned long long lo[7] = {0, 0, 0, 0, 0, 0, 0}, hi[7] = {0, 0, 0, 0, 0, 0, 0};
         int32_t w[64];
         if (WIDE) {
-            // k = 32: rows lie 2^26 bytes apart, so the wave's 64 rows span
-            // under 2^32 bytes from its first; one buffer resource per wave
-            // (p is the wave index), the row offset a scalar, the column one
-            // 32-bit offset: no 64-bit address per load to hold in VGPRs
-            const uint32_t pw = __builtin_amdgcn_readfirstlane(p);
-            const In *row0 = src + pass_index(tau, tbit, pw * 64, 0);
-            const uint32_t row_bytes = (uint32_t)sizeof(In) << tbit;
-            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
-                const_cast<In *>(row0), 0, (int)(63u * row_bytes + 64u * sizeof(In)), 0x00020000);
-#pragma unroll
-            for (int j = 0; j < 64; ++j)
-                w[j] = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)(c * sizeof(In)), (int)(j * row_bytes), 0);
-            wht64(w);  // |w| <= 2^30
+            load_rows<In>(src, tau, tbit, c, p, w);  // k = 32: rows 2^26 bytes apart
+            wht64(w);                                // |w| <= 2^30
             int32_t o[64];
             exchange_gather(w, o, E, c, p);
 #pragma unroll
