@@ -72,7 +72,10 @@ bool spectral_preferred(uint64_t A, uint64_t num_words, uint32_t *k) {
     uint64_t Ao = 0;
     uint32_t n = 0;
     if (!spectral_shape(A, kk, &Ao, &n)) return false;
-    const double spectral_s = std::ldexp(2.2e-12, (int)n) + 50e-6;
+    // measured seconds per transformed element: two passes up to k = 24
+    // (2^40 elements in 2.35 s), three beyond (2^38 in 1.03 s at k = 32)
+    const double per_element = kk <= 24 ? 2.2e-12 : 3.8e-12;
+    const double spectral_s = std::ldexp(per_element, (int)n) + 50e-6;
     const double pair_s = std::ldexp(1.0, 2 * (int)kk - 1) / 9.6e12;
     *k = kk;
     return spectral_s < 0.8 * pair_s;
