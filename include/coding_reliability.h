@@ -103,7 +103,7 @@ cr_status cr_an_pairs_tuned(cr_context *ctx, uint64_t A, uint64_t num_words, uin
  * (distance_distribution/src/an_coding.cu:50-105), exact integer arithmetic.
  * Supported: 14 <= k <= 32 and n <= min(2k, 48) (CR_ERR_UNSUPPORTED
  * otherwise); large k gives exact distributions where the pairs are out of
- * reach (A = 61: k = 28 in 66 ms, k = 32 in 1.3 s).
+ * reach (A = 61: k = 28 in 65 ms, k = 32 in 1.03 s).
  * kernel_ms (may be NULL) receives the device time of all the launches. */
 cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t k, uint64_t *raw, uint32_t nbins,
                                double *kernel_ms);
