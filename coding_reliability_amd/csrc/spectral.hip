@@ -88,13 +88,25 @@ __device__ __forceinline__ uint32_t held_row(int i, uint32_t p) {
     return 64u * (i & 3) + 16u * (i >> 4) + 4u * p + ((i >> 2) & 3);
 }
 
+// High part of the codeword with low bits x_lo + 1, from h = high part at
+// x_lo (both below A): the word is A * i with i = x_lo * Ainv mod 2^k, and
+// A * Ainv = 1 + tc * 2^k, so stepping x_lo adds 1 + tc * 2^k to the word
+// and subtracts A * 2^k when i wraps; the low k bits x_lo + 1 do not carry
+// (a thread's 64 consecutive x_lo start 64-aligned).  Hence h' = h + tc - A
+// exactly when h + tc >= A, i.e. h' = (h + tc) mod A (tc < A since Ainv <
+// 2^k): min(t, t - A) in 32 bits (h < 2^(n-k) <= 2^24 and tc < A < 2^25
+// for the supported shapes, n <= min(2k, 48), so t never wraps).
+__device__ __forceinline__ uint32_t next_high(uint32_t h, uint32_t tc, uint32_t A) {
+    const uint32_t t = h + tc;
+    return min(t, t - A);
+}
+
 // Pass 1: generates the +-1 sequence of chunk s = s0 + blockIdx.y and
 // transforms bits 0..7 of the index.  Tile: 64 transforms (index bits 8..)
 // of 256 contiguous elements, thread column c = transform.  Element x_lo of
 // the chunk holds (-1)^parity(s & h(x_lo)), h(x_lo) = (A * (x_lo * Ainv mod
-// 2^k)) >> k < 2^(n-k) <= 2^24, advanced incrementally along x_lo:
-// i += Ainv (mod 2^k), h += tc - A * wrap (mod 2^32), where A * Ainv = 1 +
-// tc * 2^k.  The transformed tile is staged in LDS (transform c, row r at
+// 2^k)) >> k < 2^(n-k) <= 2^24, advanced along x_lo by next_high.  The
+// transformed tile is staged in LDS (transform c, row r at
 // c * 256 + (r ^ c): conflict free both ways) and written row by row.
 template <class Out>
 __global__ __launch_bounds__(kThreads) void spec_gen(Out *__restrict__ X, uint32_t k, uint32_t A, uint32_t Ainv,
@@ -105,16 +117,13 @@ __global__ __launch_bounds__(kThreads) void spec_gen(Out *__restrict__ X, uint32
     const uint32_t mask = (1u << k) - 1;
     const uint32_t g = blockIdx.x * kCols + c;  // transform: index bits 8..
     const uint32_t x0 = (g << 8) | (p << 6);    // first of this thread's 64 consecutive x_lo
-    uint32_t i = (uint32_t)((uint64_t)x0 * Ainv) & mask;
+    const uint32_t i = (uint32_t)((uint64_t)x0 * Ainv) & mask;
     uint32_t h = (uint32_t)(((uint64_t)A * i) >> k);
     int32_t v[64];
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
         v[j] = 1 - 2 * (int32_t)(__popc(h & s) & 1);
-        const uint32_t ni = i + Ainv;  // < 2^25: k <= 24
-        const bool wrap = ni > mask;
-        i = ni & mask;
-        h = h + tc - (wrap ? A : 0u);
+        h = next_high(h, tc, A);
     }
     wht64(v);
     exchange_m(v, E, c, p);
@@ -144,16 +153,13 @@ __global__ __launch_bounds__(kThreads16) void spec_gen16(int32_t *__restrict__ X
     const uint32_t s = s0 + blockIdx.y;
     const uint32_t G = blockIdx.x;
     const uint32_t x0 = (G << 16) | (w << 12) | (l << 6);
-    uint32_t i = (uint32_t)((uint64_t)x0 * Ainv) & mask;
+    const uint32_t i = (uint32_t)((uint64_t)x0 * Ainv) & mask;
     uint32_t h = (uint32_t)(((uint64_t)A * i) >> k);
     int32_t v[64];
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
         v[j] = 1 - 2 * (int32_t)(__popc(h & s) & 1);
-        const uint32_t ni = i + Ainv;
-        const bool wrap = ni > mask || ni < i;  // carry out of 32 bits at k = 32
-        i = ni & mask;
-        h = h + tc - (wrap ? A : 0u);
+        h = next_high(h, tc, A);
     }
     wht64(v);  // bits 0-5; |v| <= 64
     // transpose within the wave: element (row l, column j) -> thread (w, j)

@@ -73,3 +73,25 @@ def test_krawtchouk_orthogonality():
         for e in range(n + 1):
             s = sum(comb(n, w) * krawtchouk(n, d, w) * krawtchouk(n, e, w) for w in range(n + 1))
             assert s == ((1 << n) * comb(n, d) if d == e else 0)
+
+
+@pytest.mark.parametrize("A,k", [(1, 14), (61, 16), (58659, 24), (641, 20), (61, 32), (3, 32), (16777215, 24),
+                                 (16777217, 24), (1021, 17)])
+def test_high_part_recurrence(A, k):
+    # the first pass steps the high part h(x) = (A * (x * A^-1 mod 2^k)) >> k
+    # along each 64-aligned block of x as h' = min(t, t - A) in 32-bit
+    # arithmetic, t = h + tc, A * A^-1 = 1 + tc * 2^k; checked here against the
+    # definition on blocks at the start, middle and end of the range
+    M = 1 << k
+    Ainv = pow(A, -1, M)
+    tc = (A * Ainv - 1) >> k
+    assert A * Ainv == 1 + tc * M and tc < A
+    n = max((A * (M - 1)).bit_length(), k)
+    if n > min(2 * k, 48):
+        pytest.skip("shape outside the spectral path")
+    for x0 in (0, 64 * 12345 % M, M // 2, M - 64):
+        h = (A * ((x0 * Ainv) % M)) >> k
+        for j in range(64):
+            assert h == (A * (((x0 + j) * Ainv) % M)) >> k, (x0, j)
+            t = (h + tc) & 0xFFFFFFFF
+            h = min(t, (t - A) & 0xFFFFFFFF)
