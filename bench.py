@@ -272,8 +272,9 @@ def run_spectral(ctx, A: int, k: int, raw_pairs: list, pair_kernel_s: float) -> 
         achieved = elements * bytes_per_element / (ms / 1e3) / 1e12
         memory = {"bytes_per_element": bytes_per_element, "achieved": round(achieved, 3), "peak": HBM_PEAK_TBS,
                   "unit": "TB/s", "frac": round(achieved / HBM_PEAK_TBS, 4),
-                  "note": "the first pass is bound by its instruction work (the +-1 generation and 16 butterfly "
-                          "stages), the last by memory"}
+                  "note": "the first pass (the +-1 generation and 16 butterfly stages) issues VALU at about half "
+                          "the one-per-clock rate, one 1024-thread workgroup per CU idling at its barriers; the "
+                          "last is bound by memory"}
     return {
         "method": "Walsh-Hadamard spectrum of the code: 2^(n-k) transforms of length 2^k, exact integer sums, "
                   "Krawtchouk transform on the host",
