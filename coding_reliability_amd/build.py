@@ -12,7 +12,6 @@ import concurrent.futures as cf
 import glob
 import os
 import subprocess
-import sys
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -99,13 +98,22 @@ def build_reference_cli() -> str | None:
     return REF_CLI
 
 
-def dump_isa(src_name: str, out_dir: str) -> str:
-    """Writes the gfx950 assembly of one source file (for inspection)."""
+TESTS_CPP = os.path.join(ROOT, "tests", "cpp")
+
+
+def build_test_programs() -> list[str]:
+    """C++ callers of the reference entry points used by the GPU tests
+    (tests/cpp/*.cpp, compiled against include/cr_reference_api.h and linked
+    against libcodrel.so), into _build/tests/."""
+    out_dir = os.path.join(BUILD, "tests")
     os.makedirs(out_dir, exist_ok=True)
-    src = os.path.join(CSRC, src_name)
-    out = os.path.join(out_dir, src_name + ".s")
-    cmd = [HIPCC, *COMMON, "-x", "hip", "--cuda-device-only", "-S", src, "-o", out]
-    subprocess.run(cmd, check=True)
+    out = []
+    for src in sorted(glob.glob(os.path.join(TESTS_CPP, "*.cpp"))):
+        exe = os.path.join(out_dir, os.path.splitext(os.path.basename(src))[0])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-L", PKG,
+               "-lcodrel", "-Wl,-rpath,$ORIGIN/../../coding_reliability_amd", "-o", exe]
+        subprocess.run(cmd, check=True)
+        out.append(exe)
     return out
 
 
@@ -113,5 +121,4 @@ if __name__ == "__main__":
     build(verbose=True)
     build_clis()
     build_reference_cli()
-    if len(sys.argv) > 2 and sys.argv[1] == "--isa":
-        print(dump_isa(sys.argv[2], os.path.join(ROOT, "_work", "isa")))
+    build_test_programs()

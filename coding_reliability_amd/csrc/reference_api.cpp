@@ -27,6 +27,13 @@ int Statistics::add(const std::string &label, const std::string &unit) {
 
 void Statistics::set(int i, double value) { rows_.at(i).value = value; }
 
+void Statistics::check_index(int i) const {
+    if (i < 0 || i >= getLength()) {
+        fprintf(stderr, "Index out of range (%d/%d).\n", i, getLength());
+        throw std::out_of_range("");
+    }
+}
+
 namespace {
 
 const int kOffset = 2;  // first bin of the error columns (globals.cpp:24)
@@ -630,20 +637,177 @@ double run_hamming_mc(uintll_t n, int with_1bit, uintll_t iterations, int file_o
     return max_abs;
 }
 
-static void not_provided(const char *fn) {
-    throw std::runtime_error(std::string(fn) +
-                             ": the reference CPU implementations and cuRAND dumps are not part of this GPU library");
+// ---------------------------------------------------------------------------
+// Entry points of the reference's CPU implementations, computed on the GPU
+// with those paths' own conventions.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Timing table of the CPU drivers: one row, "Total Runtime" (an_coding.cpp:106-109,
+// hamming.cpp:96-99, hamming_native*.cpp).
+Statistics cpu_stats(double total_ms) {
+    Statistics s;
+    const int it = s.add("Total Runtime", "s");
+    s.set(it, total_ms);
+    s.setFactorAll(0.001);
+    return s;
 }
 
-void run_ancoding_cpu(uintll_t, uintll_t, int, double *, uintll_t *, uint128_t *, int) {
-    not_provided("run_ancoding_cpu");
+// Reported-form Extended Hamming distribution of hamming_cpu<n> (hamming.cpp:43-93)
+// for n in {8, 16, 24, 32, 40}: the non-extended weights of all 2^n data
+// words, merged, with the 1-bit spheres, bins >= 3 shifted by n.  apply_w<N>
+// (hamming.cpp:25-41) takes 4, 5 or 6 of the 32-bit parity masks; its masks
+// see only the low 32 data bits, so for n = 40 the weight of a word is that
+// of its low 32 bits under the (39,32) code plus the popcount of its high 8
+// bits, and the distribution is the (39,32) one convolved with C(8, j).
+void hamming_cpu_counts(cr_context *ctx, uint_t n, int with_1bit, uint128_t *counts, uint_t count_counts) {
+    const uint_t k = n == 40 ? 32 : n;
+    const uint_t N = k + cr_hamming_parity_bits(k) + 1;
+    std::vector<uint64_t> w(N + 1, 0);
+    cr_ham_request req{};
+    req.k = k;
+    req.source = CR_HAM_RANGE;
+    req.first = 0;
+    req.count = 1ull << k;
+    check(cr_hamming_weights(ctx, &req, w.data(), N + 1, nullptr), "run_hamming_cpu");
+    if (n == 40) {
+        std::vector<uint64_t> w40(N + 8 + 1, 0);
+        for (uint_t a = 0; a <= N; ++a)
+            for (uint_t j = 0; j <= 8; ++j) w40[a + j] += w[a] * binom<uint64_t>(8, j);
+        w.swap(w40);
+    }
+    const uint_t NN = n + cr_hamming_parity_bits(n) + 1;
+    std::vector<uint64_t> rep(2 * (NN + 1));
+    check(cr_hamming_report(n, w.data(), with_1bit, rep.data()), "run_hamming_cpu");
+    for (uint_t i = 0; i < count_counts && i <= NN; ++i) counts[i] = ((uint128_t)rep[2 * i + 1] << 64) | rep[2 * i];
 }
-void run_hamming_cpu(uintll_t, int, int) { not_provided("run_hamming_cpu"); }
-void run_hamming_cpu_native(uintll_t, int, int) { not_provided("run_hamming_cpu_native"); }
-void run_hamming_cpu_native_short(uintll_t, int, int) { not_provided("run_hamming_cpu_native_short"); }
-void run_hamming_cpu_native_short_ordering(uintll_t, int, int) {
-    not_provided("run_hamming_cpu_native_short_ordering");
+
+// The brute-force drivers (hamming_native.cpp, hamming_native_short.cpp,
+// hamming_native_short_ordering.cpp) instantiate their templates for 8, 16
+// and, for every other n, 24 data bits, and fill the CNT_COUNTS = 8+13+1,
+// 16+6+1 or 24+6+1 bins of that code.  By linearity their ordered-pair and
+// sphere counts equal hamming_cpu's reported form (every codeword sees the
+// same distance distribution); the ordering variant counts pairs a < b
+// twice, so the diagonal's bins 0 and 1 stay zero.  Returns the code width
+// used; throws where the reference would write past its n+h+1 bins.
+uint_t hamming_native_counts(cr_context *ctx, uint_t n, int with_1bit, bool ordering, uint128_t *counts,
+                             uint_t count_counts) {
+    const uint_t code = (n == 8 || n == 16) ? n : 24;
+    const uint_t cc = code + (code == 8 ? 5 : 6) + 1;
+    if (cc > count_counts)
+        throw std::runtime_error("run_hamming_cpu_native*: n=" + std::to_string(n) +
+                                 " selects the 24-bit code, whose 31 bins exceed the n+h+1 = " +
+                                 std::to_string(count_counts) + " the reference allocates");
+    hamming_cpu_counts(ctx, code, with_1bit, counts, cc);
+    if (ordering) counts[0] = counts[1] = 0;
+    return code;
 }
+
+}  // namespace
+
+void run_ancoding_cpu(uintll_t n, uintll_t A, int verbose, double *times, uintll_t *minb, uint128_t *mincb,
+                      int file_output) {
+    // an_coding.cpp:104-142: count_counts = n + ceil(log2 A) + 1 with A as
+    // given (no power-of-two mapping), counts[0] = 2^n (the diagonal,
+    // an_coding.cpp:69) and counts[b] = 2 * #{i < j : d = b}.  For n < 6 the
+    // reference's 64-word shard walks past 2^n and writes beyond its bins; here
+    // the 2^n words of the code are used.
+    if (A == 0) throw std::invalid_argument("run_ancoding_cpu: A must be >= 1");
+    const uintll_t count_counts = n + (uintll_t)ceil(log((double)A) / log(2.0)) + 1;
+    const int dev = 0;
+    Lease lease(dev);
+    WallClock wc;
+    std::vector<uint64_t> raw(std::max<uintll_t>(count_counts, 65), 0);
+    check(cr_an_pairs(lease.ctx, A, 1ull << n, raw.data(), (uint32_t)raw.size(), nullptr), "run_ancoding_cpu");
+    std::vector<uint128_t> counts(count_counts, 0);
+    counts[0] = 1ull << n;
+    for (uintll_t i = 1; i < count_counts; ++i) counts[i] = static_cast<uint128_t>(raw[i]) << 1;
+    Statistics stats = cpu_stats(wc.ms());
+    if (verbose || file_output)
+        process_result_ancoding(counts.data(), stats, (uint_t)n, (uint_t)A, file_output ? "ancoding_cpu" : nullptr);
+    if (minb != nullptr && mincb != nullptr) {
+        // an_coding.cpp:125-137: the CPU driver starts from mincb = 0xFFFFFFFF
+        *minb = 0xFFFF;
+        *mincb = 0xFFFFFFFF;
+        for (uint_t i = 1; i < count_counts / 2; ++i)
+            if (counts[i] != 0 && counts[i] < *mincb) {
+                *minb = i;
+                *mincb = counts[i];
+                break;
+            }
+    }
+    if (times != NULL) {
+        // an_coding.cpp:138-142: the table has one row, so the second read
+        // throws std::out_of_range as in the reference
+        times[0] = stats.getAverage(0);
+        times[1] = stats.getAverage(1);
+    }
+}
+
+void run_hamming_cpu(uintll_t n, int with_1bit, int file_output) {
+    // hamming.cpp:94-126
+    const uintll_t h = n == 8 ? 5 : (n < 32 ? 6 : 7);
+    std::vector<uint128_t> counts(n + h + 1, 0);
+    printf("Start Hamming Coding Algorithm (CPU)\n");
+    Lease lease(0);
+    WallClock wc;
+    if (n == 8 || n == 16 || n == 24 || n == 32 || n == 40)
+        hamming_cpu_counts(lease.ctx, (uint_t)n, with_1bit, counts.data(), (uint_t)(n + h + 1));
+    Statistics stats = cpu_stats(wc.ms());
+    process_result_hamming(counts.data(), stats, (uint_t)n, (uint_t)h, file_output ? "hamming_cpu" : nullptr);
+}
+
+void run_hamming_cpu_native_short(uintll_t n, int with_1bit, int file_output) {
+    // hamming_native_short.cpp:86-122
+    const uintll_t h = n == 8 ? 5 : (n < 32 ? 6 : 7);
+    std::vector<uint128_t> counts(n + h + 1, 0);
+    printf("Start Hamming Coding Algorithm - Native Short Approach (CPU)\n");
+    Lease lease(0);
+    WallClock wc;
+    hamming_native_counts(lease.ctx, (uint_t)n, with_1bit, false, counts.data(), (uint_t)(n + h + 1));
+    Statistics stats = cpu_stats(wc.ms());
+    process_result_hamming(counts.data(), stats, (uint_t)n, (uint_t)h,
+                           file_output ? "hamming_cpu_native_short" : nullptr);
+}
+
+void run_hamming_cpu_native_short_ordering(uintll_t n, int with_1bit, int file_output) {
+    // hamming_native_short_ordering.cpp:85-118
+    const uintll_t h = n == 8 ? 5 : (n < 32 ? 6 : 7);
+    std::vector<uint128_t> counts(n + h + 1, 0);
+    printf("Start Hamming Coding Algorithm - Native Short Approach with Ordering (CPU)\n");
+    Lease lease(0);
+    WallClock wc;
+    hamming_native_counts(lease.ctx, (uint_t)n, with_1bit, true, counts.data(), (uint_t)(n + h + 1));
+    Statistics stats = cpu_stats(wc.ms());
+    process_result_hamming(counts.data(), stats, (uint_t)n, (uint_t)h,
+                           file_output ? "hamming_cpu_native_short_ordering" : nullptr);
+}
+
+void run_hamming_cpu_native(uintll_t n, int with_1bit, int file_output) {
+    // hamming_native.cpp:130-155; with spheres the reference also prints its
+    // "#Distances" table (hamming_native.cpp:109-118)
+    const uintll_t h = n == 8 ? 5 : (n < 32 ? 6 : 7);
+    std::vector<uint128_t> counts(n + h + 1, 0);
+    printf("Start Hamming Coding Algorithm - Native Approach (CPU)\n");
+    Lease lease(0);
+    WallClock wc;
+    const uint_t code = hamming_native_counts(lease.ctx, (uint_t)n, with_1bit, false, counts.data(),
+                                              (uint_t)(n + h + 1));
+    if (with_1bit) {
+        const uint_t cc = code + (code == 8 ? 5 : 6) + 1;
+        cout << dec << "\n#Distances:\n";
+        for (uint_t i = 0; i < cc; ++i)
+            cout << "  " << right << setw(2) << i << ": " << setw(13) << static_cast<uintll_t>(counts[i]) << '\n';
+    }
+    Statistics stats = cpu_stats(wc.ms());
+    process_result_hamming(counts.data(), stats, (uint_t)n, (uint_t)h, file_output ? "hamming_cpu_native" : nullptr);
+}
+
+static void not_provided(const char *fn) {
+    throw std::runtime_error(std::string(fn) + ": the cuRAND dumps are not part of this GPU library");
+}
+
 void test_curand(uintll_t, uintll_t, int) { not_provided("test_curand"); }
 void test_curand_1d(uintll_t, uintll_t, int) { not_provided("test_curand_1d"); }
 void test_curand_raw(uintll_t, uintll_t, int) { not_provided("test_curand_raw"); }

@@ -11,9 +11,12 @@
  * one GPU (nr_dev and Flags::nr_dev/dev select device 0 / the given index),
  * no cudaDeviceReset, HIP-event kernel timing, Monte Carlo words from
  * Philox4x32-10 instead of cuRAND, errors reported as std::runtime_error
- * instead of exit().  The reference CPU implementations (run_*_cpu*) and the
- * cuRAND dump (test_curand*) are not part of this library: calling them
- * throws std::runtime_error.
+ * instead of exit().  The entry points of the reference's CPU implementations
+ * (run_ancoding_cpu, run_hamming_cpu, run_hamming_cpu_native*) compute the
+ * same numbers on the GPU and keep those paths' own conventions (no
+ * power-of-two mapping of A, their bin counts, file prefixes and one-row
+ * timing tables).  The cuRAND dumps (test_curand*) are not part of this
+ * library: calling them throws std::runtime_error.
  */
 #ifndef CR_REFERENCE_API_H_
 #define CR_REFERENCE_API_H_
@@ -39,9 +42,16 @@ class Statistics {
     int getLength() const { return (int)rows_.size(); }
     std::string getLabel(int i) const { return rows_.at(i).label; }
     std::string getUnit(int i) const { return rows_.at(i).unit; }
-    double getAverage(int i) const { return rows_.at(i).value * factor_; }
+    double getAverage(int i) const {
+        check_index(i);
+        return rows_[i].value * factor_;
+    }
 
   private:
+    // As the helper library's Statistics::check_index
+    // (distance_distribution/lib/helper/src/statistics.cpp:165-170): a message
+    // on stderr, then std::out_of_range.
+    void check_index(int i) const;
     struct Row {
         std::string label, unit;
         double value = 0;

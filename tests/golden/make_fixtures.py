@@ -14,7 +14,9 @@ Sources (paths relative to the reference root):
   distance_distribution/results/superAs/*.txt        super-A sweep summaries
   distance_distribution/legacy/print_codeword_matrix.out   k=5 full histograms
   results/runtimes/4/ancoding_gpu_A61_n16_h6.csv, results/converge/geom/
-    ancoding_grid1d_gpu_m1001_A61_n16_h6.csv        CSV text (row and summary lines)
+    ancoding_grid1d_gpu_m1001_A61_n16_h6.csv,
+  results/runtimes/4/ancoding_cpu_A61_n{8,16,24}_h6.csv
+                                                     CSV text (row and summary lines)
   distance_distribution/legacy/hamming_count_undetectable_errors.cpp
                                                      (40,33) and (52,45) weight tables
 
@@ -176,6 +178,9 @@ def recorded_csv_text():
         "exhaustive_A61_n16": "results/runtimes/4/ancoding_gpu_A61_n16_h6.csv",
         "grid1d_m1001_A61_n16": "results/converge/geom/ancoding_grid1d_gpu_m1001_A61_n16_h6.csv",
     }
+    # the reference CPU path's own files (run_ancoding_cpu: ancoding_cpu prefix, one timing row)
+    for n in (8, 16, 24):
+        files[f"cpu_A61_n{n}"] = f"results/runtimes/4/ancoding_cpu_A61_n{n}_h6.csv"
     for A in (1, 3, 17, 63, 99, 127):
         for n in (8, 12, 16):
             for f in glob.glob(os.path.join(DD, "results", "As", f"ancoding_gpu_A{A}_n{n}_h*.csv")):

@@ -254,3 +254,13 @@ def test_an_candidates_bruteforce(k, abits):
             for A in range((1 << (abits - 1)) + 1, 1 << abits, 2)}
     assert best == max(vals.values())
     assert cands == sorted(A for A, v in vals.items() if v == best)
+
+
+def test_reported_form_from_macwilliams_equals_sphere_enumeration():
+    # the test helper that turns an extended weight distribution into
+    # run_hamming's reported form, pinned by the explicit sphere enumeration
+    # of hamming_native_short.cpp:52-77 (oracle brute force)
+    from conftest import hamming_reported_from_ext
+    for k in (4, 8):
+        for b in (False, True):
+            assert hamming_reported_from_ext(k, macwilliams_weights(k), b) == O.hamming_bruteforce(k, b)

@@ -177,3 +177,113 @@ def test_cli_grid_k40_csv_text_matches_recorded_file(tmp_path):
     rows, summary = _text(tmp_path / "ancoding_grid1d_gpu_m101_A61_n40_h6.csv")
     assert [",".join(r.split(",")[:4]) for r in rows] == rec["rows"], rec["src"]
     assert [x for x in summary if x.startswith('"Total"')] == rec["summary"], rec["src"]
+
+
+# --- the reference CPU implementations' entry points (served by the GPU) ------
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+@pytest.mark.parametrize("n", [8, 16, 24])
+def test_cli_cpu_path_matches_recorded_file(tmp_path, n):
+    # -c: run_ancoding_cpu (an_coding.cpp:104-142) with its own conventions:
+    # the ancoding_cpu prefix, a one-row timing table, no device line; every
+    # row (counts, probabilities, binomial bases) and the total as recorded
+    rec = golden("recorded_csv_text.json")[f"cpu_A61_n{n}"]
+    run_cli(tmp_path, "-c", "-a", "61", "-n", str(n), "-f")
+    path = tmp_path / f"ancoding_cpu_A61_n{n}_h6.csv"
+    rows, summary = _text(path)
+    assert rows == rec["rows"], rec["src"]
+    assert summary == rec["summary"], rec["src"]
+    txt = open(path).read()
+    assert '0,"Total Runtime",' in txt and "Kernel Runtime" not in txt and '"Device"' not in txt
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+def test_cli_cpu_path_keeps_power_of_two_multiplier(tmp_path):
+    # the CPU path has no power-of-two -> 1 mapping: A = 8 keeps h = 3 and
+    # n + 4 bins, with the distances of A = 1
+    from oracle import oracle as O
+    n = 10
+    run_cli(tmp_path, "-c", "-a", "8", "-n", str(n), "-f")
+    got = read_counts(tmp_path / f"ancoding_cpu_A8_n{n}_h3.csv")
+    want, _ = O.an_cpu_counts(n, 8)
+    assert len(got) == n + 4 and got == want
+
+
+def _hamming_cli(tmp_path, e, n, with_1bit, prefix, extra=()):
+    args = ["-e", str(e), "-n", str(n), "-f", *extra] + (["-b"] if with_1bit else [])
+    out = run_cli(tmp_path, *args)
+    h = 5 if n == 8 else (6 if n < 32 else 7)
+    return read_counts(tmp_path / f"{prefix}_n{n}_h{h}.csv"), out
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+@pytest.mark.parametrize("with_1bit", [0, 1])
+def test_cli_hamming_bruteforce_entry_points(tmp_path, with_1bit):
+    # -e 2 / -e 4: run_hamming_cpu_native_short / run_hamming_cpu_native
+    # (hamming_native_short.cpp:29-84, hamming_native.cpp:27-127) enumerate
+    # every ordered pair of codewords and their 1-bit spheres; the oracle's
+    # explicit enumeration of the same spheres; -e 3 counts pairs a < b twice
+    # (hamming_native_short_ordering.cpp:58-80), so its bins 0 and 1 are 0
+    from oracle import oracle as O
+    for n in (8, 16):
+        brute = O.hamming_bruteforce(n, bool(with_1bit)) if n == 8 else None
+        ref = O.hamming_report(n, O.hamming_weights(n), bool(with_1bit))
+        if brute is not None:
+            assert ref == brute
+        got, _ = _hamming_cli(tmp_path, 2, n, with_1bit, "hamming_cpu_native_short")
+        assert got == ref, n
+        got, out = _hamming_cli(tmp_path, 4, n, with_1bit, "hamming_cpu_native")
+        assert got == ref, n
+        assert ("#Distances:" in out) == bool(with_1bit)
+        got, _ = _hamming_cli(tmp_path, 3, n, with_1bit, "hamming_cpu_native_short_ordering")
+        assert got == [0, 0] + ref[2:], n
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+@pytest.mark.parametrize("n", [8, 16, 24, 32])
+def test_cli_hamming_cpu_entry_point(tmp_path, n, solutions):
+    # -e 1 -c: run_hamming_cpu (hamming.cpp:94-126): even bins are the
+    # solutions.h weight distributions << n, odd bins the sphere closed form
+    from conftest import hamming_reported_from_ext, macwilliams_weights
+    out = run_cli(tmp_path, "-e", "1", "-c", "-n", str(n), "-b", "-f")
+    assert "Start Hamming Coding Algorithm (CPU)" in out
+    want = hamming_reported_from_ext(n, macwilliams_weights(n), True)
+    sol = solutions[f"sol_hamming_n{n}"]
+    assert all(want[i] == sol[i] << n for i in range(4, len(sol), 2))
+    h = 5 if n == 8 else (6 if n < 32 else 7)
+    _check_wide_counts(tmp_path / f"hamming_cpu_n{n}_h{h}.csv", want)
+
+
+def _check_wide_counts(path, want):
+    # counts above 2^64 are printed with 12 significant digits when n >= 32
+    # (globals.cpp:204-207), every other count exactly
+    rows = []
+    for line in open(path):
+        if not line.strip():
+            break
+        rows.append(line.split(",")[1].strip())
+    assert len(rows) == len(want)
+    for i, (r, w) in enumerate(zip(rows, want)):
+        if w > 1 << 64:
+            assert abs(float(r) / w - 1) < 1e-11, i
+        else:
+            assert int(r) == w, i
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+def test_cli_hamming_cpu_entry_point_n40(tmp_path):
+    # run_hamming_cpu for n = 40: apply_w<40> (hamming.cpp:25-41) takes the
+    # six 32-bit parity masks, a (47,40) code whose distribution the
+    # MacWilliams identity of its dual gives exactly; counts above 2^64 are
+    # printed with 12 significant digits (globals.cpp:204-207)
+    from conftest import hamming_reported_from_ext, macwilliams_weights
+    n = 40
+    masks32 = [0x56AAAD5B, 0x9B33366D, 0xE3C3C78E, 0x03FC07F0, 0x03FFF800, 0xFC000000]
+    want = hamming_reported_from_ext(n, macwilliams_weights(n, masks32), True)
+    run_cli(tmp_path, "-e", "1", "-c", "-n", str(n), "-b", "-f")
+    _check_wide_counts(tmp_path / f"hamming_cpu_n{n}_h7.csv", want)
