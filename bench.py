@@ -9,12 +9,19 @@ input tensors: codewords are generated in registers, so the inputs are
 trivially resident before the timed region starts.
 
 Prints ONE JSON line (rank 0).  Extra keys beyond the required ones:
-  roofline       VALU roofline of the pair kernel (events measured live)
+  roofline       LDS-increment roofline of the pair kernel (its binding one)
+                 and its VALU roofline, each against a ceiling measured in
+                 this run (cr_probe_ceiling), kernel time from HIP events
+  ceilings       the measured ceilings (VALU xor+popcount, Philox, LDS increments)
   cpu_baseline   the oracle's restatement of the reference CPU loop on a
                  bounded row-shard sample of the same workload, host cores
+  spectral       the same histogram through the Walsh-Hadamard path
   batched        4096 mixed-size codes batched (one launch + captured hipGraph) vs one code per
                  launch, and the super-A k=8 sweep (skip with --no-batched)
-  served         continuous-batching request mix (with --served)
+  kernels        the other hot kernels on their workloads (skip with --no-kernels)
+  served         continuous-batching request mix with Poisson arrivals, p50/p99
+                 latency, and a burst of small requests against one per launch
+                 (skip with --no-served)
 """
 from __future__ import annotations
 
@@ -31,7 +38,6 @@ A_HEADLINE = 58659
 K_HEADLINE = 24
 PEAK_CLOCK_GHZ = 2.4
 LANES_PER_CU_CLK = 64
-LDS_ATOMIC_LANES_PER_CU_CLK = 16
 HBM_PEAK_TBS = 8.0  # MI355X HBM3E, published
 
 
@@ -89,6 +95,9 @@ def cpu_baseline(A: int, k: int, target_s: float) -> dict:
         "kind": "port",
         "sample": f"A={A} k={k}: row shards 0..{shards - 1} of {nshards_total} "
                   f"(64 words each vs all later words), {pairs} pairs in {dt:.2f} s on {cpu_model()}",
+        "build": "gcc -O3 -march=x86-64-v3 -fopenmp: the oracle is built once where the tree is built and runs on "
+                 "whichever x86-64 host runs the GPU (its model is not known at build time); its hot loop is xor + "
+                 "popcount + increment, which x86-64-v3 (POPCNT, AVX2) covers",
     }
 
 
@@ -161,24 +170,49 @@ def run_batched(ctx, autotune: bool = True) -> dict:
     return out
 
 
-# Algorithmic int32 lane-ops per unit of the other kernels (VALU-bound all):
-#   Hamming weight of a sampled word: one table lookup per data byte (the
-#   byte extract; the syndrome is linear, so it is the XOR of per-byte
-#   syndromes), the XORs, popcounts of syndrome and data, one increment: 9 for
-#   32-bit words, 16 for 64-bit words; Monte Carlo adds the Philox4x32-10
-#   share (10 rounds x 6 ops per block: 15 per 32-bit word, 30 per 64-bit).  Exhaustive enumeration: the syndrome and the
-#   popcount split into a per-lane and a per-iteration part (the words of an
-#   aligned block are disjoint bit fields), leaving xor, popcount-accumulate
-#   and the increment (3) per word.  Reference grid pair: xor + popcount and an
-#   increment (3; the rows of a row block share their high dword, so its
-#   distance is one scalar per sample, folded into the popcount).  AN Monte
-#   Carlo pair: 2W + 1, the multiply, and the Philox share (15).  ANcandidates step (A, d): c += A, 3c, xor, popcount on
-#   64 bits (8) and a min (1).
-def _kernel_entry(units: float, ms: float, ops: float, unit: str, peak: float, workload: str) -> dict:
+# Work per unit of the other kernels, in three classes, each measured against
+# its own ceiling (cr_probe_ceiling, run in the same process):
+#   valu    int32 lane-ops on the VALU, the histogram increment counted as one
+#           (it costs the LDS address computation), as the pair kernel's
+#           2W+1 does;
+#   philox  Philox4x32-10 lane-ops, 60 per block: 15 per 32-bit word drawn
+#           (4 words per block), 30 per 64-bit word;
+#   lds     histogram increments (one wave-wide ds_add_u32 lane each).
+# Per unit: Hamming weight of an enumerated word: xor, popcount-accumulate,
+# increment (the syndrome and popcount split into per-lane and per-iteration
+# parts over an aligned block), 3 / 0 / 1.  A sampled 32-bit word: 4 byte
+# extracts for the (linear) syndrome's per-byte tables, 3 xors, 2 popcounts,
+# 9 / 0 / 1 (64-bit: 16 / 0 / 1); Monte Carlo adds the generator, 15 or 30.
+# Reference grid pair: xor + popcount + increment, 3 / 0 / 1 (a row block's
+# common high dword folds into the popcount).  AN Monte Carlo pair: 2W + 1
+# and the multiply of the sample by A, 4 / 15 / 1.  ANcandidates step (A, d):
+# c += A, 3c, xor and popcount on 64 bits, a min: 9 / 0 / 0.
+# The VALU fraction adds the two VALU classes' times (valu/P_valu +
+# philox/P_philox per unit); the reported frac is the larger of it and the
+# LDS fraction (the two pipelines run concurrently), named by "bound".
+def _kernel_entry(units: float, ms: float, work: tuple, unit: str, ceil: dict, workload: str) -> dict:
+    valu, philox, lds = work
     rate = units / (ms / 1e3)
-    achieved = rate * ops / 1e12
+    t_valu = valu / ceil["valu"] + philox / ceil["philox"]
+    f_valu = rate * t_valu
+    f_lds = rate * lds / ceil["lds"] if lds else 0.0
+    bound = "valu" if f_valu >= f_lds else "lds_atomic"
     return {"workload": workload, "kernel_ms": round(ms, 3), "rate": round(rate / 1e9, 3), "rate_unit": unit,
-            "ops_per_unit": ops, "achieved_tops": round(achieved, 3), "frac": round(achieved / peak, 4)}
+            "work_per_unit": {"valu": valu, "philox": philox, "lds_increments": lds},
+            "achieved_tops": round(rate * (valu + philox) / 1e12, 3),
+            "frac_valu": round(f_valu, 4), "frac_lds": round(f_lds, 4), "bound": bound,
+            "frac": round(max(f_valu, f_lds), 4)}
+
+
+def measure_ceilings(ctx) -> dict:
+    """The device ceilings of this run (cr_probe_ceiling): best of three
+    launches of each probe after one untimed launch."""
+    iters = {0: 1 << 14, 1: 1 << 11, 2: 1 << 14}
+    out = {}
+    for key, probe in (("valu", 0), ("philox", 1), ("lds", 2)):
+        ctx.probe_ceiling(probe, iters[probe])
+        out[key] = max(ctx.probe_ceiling(probe, iters[probe])[0] for _ in range(3))
+    return out
 
 
 def _warm(f):
@@ -188,7 +222,7 @@ def _warm(f):
     return f()
 
 
-def run_kernels(ctx, peak: float) -> dict:
+def run_kernels(ctx, ceil: dict) -> dict:
     """The other hot kernels on their workloads, each checked for a complete
     histogram, timed with HIP events on the library's stream after one
     untimed call."""
@@ -197,11 +231,11 @@ def run_kernels(ctx, peak: float) -> dict:
     k = 32
     h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_RANGE, 0, 1 << k))
     assert sum(h) == 1 << k
-    out["hamming_exhaustive_39_32"] = _kernel_entry(1 << k, ms, 3, "Gwords/s", peak,
+    out["hamming_exhaustive_39_32"] = _kernel_entry(1 << k, ms, (3, 0, 1), "Gwords/s", ceil,
                                                     "(39,32) weights of all 2^32 data words")
     h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 20, stride=65537))
     assert sum(h) == 1 << 20
-    e = _kernel_entry(1 << 20, ms, 9, "Gwords/s", peak, "(39,32) integer grid d = i*65537 mod 2^32, i < 2^20")
+    e = _kernel_entry(1 << 20, ms, (9, 0, 1), "Gwords/s", ceil, "(39,32) integer grid d = i*65537 mod 2^32, i < 2^20")
     # 2^20 words are well under a microsecond of device work: the launch is
     # latency-bound, so its floor (the same kernel over 4096 words) is given
     _, floor_ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 12, stride=65537))
@@ -210,28 +244,26 @@ def run_kernels(ctx, peak: float) -> dict:
     S = 1 << 34
     h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_MC, 0, S, seed=2026, stream=0))
     assert sum(h) == S
-    out["hamming_mc_39_32"] = _kernel_entry(S, ms, 24, "Gwords/s", peak, "(39,32) 2^34 Philox words")
+    out["hamming_mc_39_32"] = _kernel_entry(S, ms, (9, 15, 1), "Gwords/s", ceil, "(39,32) 2^34 Philox words")
     S = 1 << 30
     h, ms = _warm(lambda: ctx.hamming_weights(64, cra.HAM_MC, 0, S, seed=2026, stream=1))
     assert sum(h) == S
-    out["hamming_mc_72_64"] = _kernel_entry(S, ms, 46, "Gwords/s", peak, "(72,64) 2^30 Philox words")
+    out["hamming_mc_72_64"] = _kernel_entry(S, ms, (16, 30, 1), "Gwords/s", ceil, "(72,64) 2^30 Philox words")
     h, ms = _warm(lambda: ctx.an_grid_ref(1, 32, 1001, 0, 61))
     pairs = sum(h)
-    out["an_grid1d_ref"] = _kernel_entry(pairs, ms, 3, "Gpairs/s", peak,
+    out["an_grid1d_ref"] = _kernel_entry(pairs, ms, (3, 0, 1), "Gpairs/s", ceil,
                                          f"reference grid 1D, A=61, k=32, m=1001 (38-bit codewords, {pairs} pairs)")
     m = 1 << 16
     h, ms = _warm(lambda: ctx.an_mc(61, 24, m, 16384))
     assert sum(h) == (1 << 24) * m
-    out["an_mc"] = _kernel_entry((1 << 24) * m, ms, 19, "Gpairs/s", peak, "AN Monte Carlo, A=61, k=24, 2^16 samples")
-    out["an_mc"]["note"] = ("frac can exceed 1: this Philox-heavy mix issues about 1.04 VALU wave-instructions per "
-                            "CU per clock (SQ_INSTS_VALU over active cycles), above the one-per-clock issue the "
-                            "64 lanes x 2.4 GHz peak assumes; 18.0 VALU per pair executed")
+    out["an_mc"] = _kernel_entry((1 << 24) * m, ms, (4, 15, 1), "Gpairs/s", ceil,
+                                 "AN Monte Carlo, A=61, k=24, 2^16 samples")
     best, cands, ms = _warm(lambda: ctx.an_candidates(24, 16))
     # pruning skips most (A, d) steps, so the work is counted as the steps the
     # candidates (A attaining the maximum, never pruned) must run in full: a
     # lower bound of the executed steps, hence a lower bound of the fraction
     steps = len(cands) * ((1 << 24) - 1)
-    e = _kernel_entry(steps, ms, 9, "Gsteps/s", peak,
+    e = _kernel_entry(steps, ms, (9, 0, 0), "Gsteps/s", ceil,
                       "ANcandidates cell k=24, |A|=16 (16384 odd A); rate = steps of the A attaining the maximum "
                       "(each runs all 2^24-1 steps) per second, a lower bound")
     e["best"], e["candidates"] = best, len(cands)
@@ -495,13 +527,15 @@ def main() -> int:
     ap.add_argument("--no-spectral", action="store_true", help="skip the spectral computation of the headline histogram")
     ap.add_argument("--spectral-only", nargs=2, type=int, metavar=("A", "K"),
                     help="one cr_an_pairs_spectral call for (A, K), checked to sum to C(2^K, 2) (for counter runs)")
-    ap.add_argument("--served", action="store_true", help="also run the served request mix")
+    ap.add_argument("--no-served", action="store_true", help="skip the served request mix")
+    ap.add_argument("--served-only", action="store_true", help="only the served request mix")
+    ap.add_argument("--ceilings-only", action="store_true", help="only the device ceiling probes")
     ap.add_argument("--no-kernels", action="store_true", help="skip the other kernels' measurements")
     ap.add_argument("--kernels-only", action="store_true", help="only the other kernels (for counter runs)")
     ap.add_argument("--vs-reference", action="store_true",
                     help="only the workloads the reference records timings for")
     ap.add_argument("--served-jobs", type=int, default=24)
-    ap.add_argument("--served-rate", type=float, default=1.0)
+    ap.add_argument("--served-rate", type=float, default=2.0)
     ap.add_argument("--traffic-probe", action="store_true", help="copy + one pair launch, for PMC counter runs")
     args = ap.parse_args()
     if args.traffic_probe:
@@ -521,9 +555,20 @@ def main() -> int:
     info = ctx.device_info()
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']} clock={info['max_clock_mhz']} MHz")
     tune = dict(rows_per_lane=args.rows_per_lane, chunk_blocks=args.chunk_blocks)
-    peak_tops = info["compute_units"] * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12
+    ceil = measure_ceilings(ctx)
+    log("ceilings: " + ", ".join(f"{kk} {v / 1e12:.2f} T/s" for kk, v in ceil.items()))
+    if args.ceilings_only:
+        print(json.dumps({"ceilings": {kk: round(v / 1e12, 3) for kk, v in ceil.items()}}), flush=True)
+        ctx.close()
+        return 0
     if args.kernels_only:
-        print(json.dumps({"kernels": run_kernels(ctx, peak_tops)}), flush=True)
+        print(json.dumps({"kernels": run_kernels(ctx, ceil)}), flush=True)
+        ctx.close()
+        return 0
+    if args.served_only:
+        served = run_served(ctx, args.served_jobs, args.served_rate)
+        served["small_burst"] = run_served_small(ctx)
+        print(json.dumps({"served": served}), flush=True)
         ctx.close()
         return 0
     if args.spectral_only:
@@ -556,36 +601,43 @@ def main() -> int:
     gpairs = pairs_per_step * args.steps / elapsed / 1e9
     avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
 
-    # VALU roofline: per pair the kernel needs one xor, one popcount and one
-    # histogram increment (3 int32 lane-ops); for 33..64-bit codewords whose
-    # column high dword is constant over 2^32/A columns, the high-dword
-    # distance is a per-segment constant folded into the popcount.
+    # Rooflines of the pair kernel, each against a ceiling measured in this
+    # run.  Per pair it executes one xor, one popcount (for 33..64-bit
+    # codewords whose column high dword is constant over 2^32/A columns, the
+    # high-dword distance is a per-segment constant folded into the
+    # popcount) and one histogram increment: 3 int32 VALU ops (the 2W+1 of
+    # W = 1; the increment costs its LDS address computation) and one
+    # wave-wide ds_add_u32 lane.  The LDS increment rate binds: one
+    # increment per pair runs at 16 lanes per clock per CU against 64 VALU
+    # lanes, so 3 VALU per pair cannot reach the VALU ceiling first.
     width_bits = ((N - 1) * A).bit_length()
     W = 1 if width_bits <= 32 else 2
     ops_per_pair = 3
     cus = info["compute_units"]
-    peak = cus * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12  # T lane-ops/s
-    achieved = pairs_per_step / avg_kernel_s * ops_per_pair / 1e12
+    nominal = cus * LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12  # T lane-ops/s
+    pair_rate = pairs_per_step / avg_kernel_s
     roofline = {
-        "bound": "valu",
-        "achieved": round(achieved, 3),
-        "peak": round(peak, 3),
-        "unit": "Tops/s (int32 lane-ops)",
-        "frac": round(achieved / peak, 4),
+        "bound": "lds_atomic",
+        "achieved": round(pair_rate / 1e12, 4),
+        "peak": round(ceil["lds"] / 1e12, 4),
+        "unit": "T LDS increments/s (one per pair)",
+        "frac": round(pair_rate / ceil["lds"], 4),
         "traffic": measured_traffic(A, k),
         "kernel": "pair_kernel<R>",
         "kernel_ms": round(avg_kernel_s * 1e3, 3),
-        "ops_per_pair": ops_per_pair,
-        "ops_per_pair_2W_plus_1": 2 * W + 1,
-        "peak_clock_ghz": PEAK_CLOCK_GHZ,
-        # the binding limit of this kernel: one LDS increment per pair, and a
-        # wave-wide ds_add_u32 holds a CU's LDS for 4 clocks (16 lanes/clk,
-        # measured with SQ_INSTS_LDS / GRBM_GUI_ACTIVE on this device)
-        "lds_atomic": {
-            "achieved": round(pairs_per_step / avg_kernel_s / 1e12, 3),
-            "peak": round(cus * LDS_ATOMIC_LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9 / 1e12, 3),
-            "unit": "T increments/s",
-            "frac": round(pairs_per_step / avg_kernel_s / (cus * LDS_ATOMIC_LANES_PER_CU_CLK * PEAK_CLOCK_GHZ * 1e9), 4),
+        "peak_source": "cr_probe_ceiling(2) in this run: wave-wide ds_add_u32 to lane-private LDS counters, "
+                       "the pair kernel's histogram layout",
+        "valu": {
+            "achieved": round(pair_rate * ops_per_pair / 1e12, 3),
+            "peak": round(ceil["valu"] / 1e12, 3),
+            "unit": "T int32 lane-ops/s",
+            "frac": round(pair_rate * ops_per_pair / ceil["valu"], 4),
+            "ops_per_pair": ops_per_pair,
+            "ops_per_pair_2W_plus_1": 2 * W + 1,
+            "peak_source": "cr_probe_ceiling(0) in this run: xor + popcount-accumulate on 8 independent chains",
+            "nominal_peak": round(nominal, 3),
+            "nominal_frac": round(pair_rate * ops_per_pair / 1e12 / nominal, 4),
+            "nominal_note": f"{cus} CU x 64 lanes x {PEAK_CLOCK_GHZ} GHz, one VALU instruction per CU per clock",
         },
     }
 
@@ -603,12 +655,12 @@ def main() -> int:
     kernels = None
     if not args.no_kernels:
         log("other kernels ...")
-        kernels = run_kernels(ctx, peak)
+        kernels = run_kernels(ctx, ceil)
         for name, e in kernels.items():
             log(f"{name}: {e['rate']} {e['rate_unit']}, frac {e['frac']}")
 
     served = None
-    if args.served:
+    if not args.no_served:
         log("served mix ...")
         served = run_served(ctx, args.served_jobs, args.served_rate)
         served["small_burst"] = run_served_small(ctx)
@@ -630,12 +682,16 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u32",
+        "dtype": "u64",
+        "dtype_note": "exact integer arithmetic: codewords of up to 64 bits in 32-bit VALU lanes (the high dword's "
+                      "distance folded per column segment), histogram counts uint64, bit-exact",
         "data": "synthetic (codewords c=A*d generated on the GPU)",
         "config": {"workload": f"exhaustive AN histogram, A={A}, k={k}, {width_bits}-bit codewords, "
                                f"all C(2^{k},2) pairs, {width_bits + 1} bins",
                    "A": A, "k": k, "pairs_per_step": pairs_per_step, "parallelism": "single GPU"},
         "roofline": roofline,
+        "ceilings": {kk: round(v / 1e12, 4) for kk, v in ceil.items()},
+        "ceilings_unit": "T/s: valu and philox int32 lane-ops, lds lane increments (cr_probe_ceiling)",
         "spectral": spectral,
         "cpu_baseline": cpu,
         "batched": batched,

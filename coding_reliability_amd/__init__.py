@@ -130,6 +130,15 @@ class Context:
         check(lib().cr_device_memory(self._h, C.byref(f), C.byref(t)))
         return f.value, t.value
 
+    PROBE_VALU, PROBE_PHILOX, PROBE_LDS = 0, 1, 2
+
+    def probe_ceiling(self, probe: int, iters: int = 1 << 14) -> tuple[float, float]:
+        """(units per second, device ms) of a ceiling probe (cr_probe_ceiling):
+        0 xor+popcount lane-ops, 1 Philox lane-ops, 2 LDS lane increments."""
+        rate, ms = C.c_double(0), C.c_double(0)
+        check(lib().cr_probe_ceiling(self._h, probe, iters, C.byref(rate), C.byref(ms)))
+        return rate.value, ms.value
+
     def an_pairs(self, A: int, num_words: int, nbins: int | None = None, rows_per_lane: int = 0,
                  chunk_blocks: int = 0, waves: int = 0) -> tuple[list[int], float]:
         """Raw unordered-pair histogram of codewords A*i, i < num_words, and the

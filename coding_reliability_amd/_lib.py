@@ -87,6 +87,8 @@ SIGNATURES = {
     "cr_context_destroy": (C.c_int, [C.c_void_p]),
     "cr_device_query": (C.c_int, [C.c_void_p, C.POINTER(cr_device_info)]),
     "cr_device_memory": (C.c_int, [C.c_void_p, u64p, u64p]),
+    "cr_probe_ceiling": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]),
     "cr_an_pairs": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, u64p, C.c_uint32, C.POINTER(C.c_double)]),
     "cr_an_pairs_tuned": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, u64p, C.c_uint32,
                                     C.POINTER(cr_pair_tuning), C.POINTER(C.c_double)]),

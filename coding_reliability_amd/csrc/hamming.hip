@@ -284,40 +284,57 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
     if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, jobs[cur].nbins, copy_out(jobs[cur]));
 }
 
-// AN Monte Carlo (n <= 32): item = (row block of 64*RM words, sample chunk
-// [j0, j0+CJ)).  Row i draws u_ij = word j of the Philox stream (seed, i):
-// block j/4 of counter (j/4, 0, i, tag), word j%4, masked to n bits; the
-// codewords A*i and A*u_ij are compared.
+// AN Monte Carlo: item = (row block of 64*RM words, sample chunk [j0,
+// j0+CJ)).  Row i draws u_ij from the Philox block b of counter
+// (b, b >> 32, i mod 2^32, tag ^ (i >> 32)), key (seed, seed >> 32):
+//   n <= 32: b = j/4, word j%4, masked to n bits;
+//   n >  32: b = j/2, words 2(j%2) (low) and 2(j%2)+1 (high), masked to n bits;
+// the codewords A*i and A*u_ij are compared.
 constexpr int RM = 4;
 constexpr uint32_t CJ = 256;  // samples per item (multiple of 4)
 
 // One AN Monte Carlo item.  W1: every codeword fits 32 bits (32-bit multiply,
-// xor and popcount).  CHECK: rows past 2^n or a partial sample chunk go to the
-// dummy row.
-template <bool W1, bool CHECK>
+// xor and popcount).  S64: 64-bit samples (n > 32).  CHECK: rows past 2^n or
+// a partial sample chunk go to the dummy row.
+template <bool W1, bool S64, bool CHECK>
 __device__ __forceinline__ void an_mc_item(uint32_t *lane_col, uint64_t A, uint64_t rb, uint64_t nwords, uint64_t j0,
-                                           uint64_t j1, uint32_t dmask, uint32_t k0, uint32_t k1, uint32_t dummy) {
+                                           uint64_t j1, uint64_t smask, uint32_t k0, uint32_t k1, uint32_t dummy) {
     const uint32_t lane = threadIdx.x;
+    constexpr uint32_t per = S64 ? 2 : 4;
 #pragma unroll
     for (int r = 0; r < RM; ++r) {
         const uint64_t i = rb * (64ull * RM) + (uint64_t)r * 64 + lane;
         const bool ok = !CHECK || i < nwords;
         const uint64_t w = A * i;
-        for (uint64_t j = j0; j < j1; j += 4) {
-            const uint64_t blk = j >> 2;
-            uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)i, kMcTag};
+        const uint32_t tag = S64 ? kMcTag ^ (uint32_t)(i >> 32) : kMcTag;
+        for (uint64_t j = j0; j < j1; j += per) {
+            const uint64_t blk = j / per;
+            uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)i, tag};
             philox4x32_10(c, k0, k1);
+            if (S64) {
 #pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) {
-                const uint32_t d = W1 ? (uint32_t)__popc((uint32_t)w ^ ((uint32_t)A * (c[u] & dmask)))
-                                      : (uint32_t)__popcll(w ^ (A * (uint64_t)(c[u] & dmask)));
-                lds_inc(lane_col + ((!CHECK || (ok && j + u < j1)) ? d : dummy) * kStride);
+                for (uint32_t u = 0; u < 2; ++u) {
+                    const uint64_t x = ((uint64_t)c[2 * u] | ((uint64_t)c[2 * u + 1] << 32)) & smask;
+                    const uint32_t d = (uint32_t)__popcll(w ^ (A * x));
+                    lds_inc(lane_col + ((!CHECK || (ok && j + u < j1)) ? d : dummy) * kStride);
+                }
+            } else {
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const uint32_t x = c[u] & (uint32_t)smask;
+                    const uint32_t d = W1 ? (uint32_t)__popc((uint32_t)w ^ ((uint32_t)A * x))
+                                          : (uint32_t)__popcll(w ^ (A * (uint64_t)x));
+                    lds_inc(lane_col + ((!CHECK || (ok && j + u < j1)) ? d : dummy) * kStride);
+                }
             }
         }
     }
 }
 
-__global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint64_t RBlocks,
+// Item t = (row group t / SC of G row blocks, sample chunk t % SC): G is
+// chosen on the host so that an item holds about 2^18 pairs whatever m is
+// (one work-queue step per item).
+__global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint32_t G,
                                                    uint64_t SC, uint64_t total_items,
                                                    unsigned long long *__restrict__ counter,
                                                    unsigned long long *__restrict__ out, uint32_t nbins,
@@ -328,29 +345,42 @@ __global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint6
     uint32_t *const lane_col = lds + lane;
     const uint32_t dummy = lds_rows - 1;
     const uint64_t nwords = 1ull << n;
-    const uint32_t dmask = (uint32_t)(nwords - 1);
+    const uint64_t RBlocks = (nwords + 64ull * RM - 1) / (64ull * RM);
+    const uint64_t smask = nwords - 1;
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     // all codewords (A * (2^n - 1)) below 2^32
     const bool w1 = ((unsigned __int128)A * (nwords - 1)) >> 32 == 0;
-    uint64_t pending = 0;
+    const bool s64 = n > 32;
+    uint64_t pending = 0;  // increments per lane counter since the last flush (bound)
     bool dirty = false;
     for (uint64_t t = first_item(); t < total_items; t = following_item(counter, t)) {
-        const uint64_t rb = uniform64(t / SC), sc = uniform64(t - (t / SC) * SC);
+        const uint64_t rg = uniform64(t / SC), sc = uniform64(t - (t / SC) * SC);
         const uint64_t j0 = sc * CJ;
         const uint64_t j1 = j0 + CJ < m ? j0 + CJ : m;
-        if ((pending + (j1 - j0)) * RM >= 0xFFFFFFFFull) {
+        const uint64_t add = (uint64_t)G * RM * (j1 - j0);
+        if (pending + add >= 0xFFFFFFFFull) {
             flush_rows(lds, 0, nbins, out);
             pending = 0;
         }
-        pending += j1 - j0;
+        pending += add;
         dirty = true;
-        const bool full = (rb + 1) * (64ull * RM) <= nwords && j1 - j0 == CJ;
-        if (w1 && full)
-            an_mc_item<true, false>(lane_col, A, rb, nwords, j0, j1, dmask, k0, k1, dummy);
-        else if (full)
-            an_mc_item<false, false>(lane_col, A, rb, nwords, j0, j1, dmask, k0, k1, dummy);
-        else
-            an_mc_item<false, true>(lane_col, A, rb, nwords, j0, j1, dmask, k0, k1, dummy);
+        for (uint32_t g = 0; g < G; ++g) {
+            const uint64_t rb = rg * G + g;
+            if (rb >= RBlocks) break;
+            const bool full = (rb + 1) * (64ull * RM) <= nwords && j1 - j0 == CJ;
+            if (s64) {
+                if (full)
+                    an_mc_item<false, true, false>(lane_col, A, rb, nwords, j0, j1, smask, k0, k1, dummy);
+                else
+                    an_mc_item<false, true, true>(lane_col, A, rb, nwords, j0, j1, smask, k0, k1, dummy);
+            } else if (w1 && full) {
+                an_mc_item<true, false, false>(lane_col, A, rb, nwords, j0, j1, smask, k0, k1, dummy);
+            } else if (full) {
+                an_mc_item<false, false, false>(lane_col, A, rb, nwords, j0, j1, smask, k0, k1, dummy);
+            } else {
+                an_mc_item<false, false, true>(lane_col, A, rb, nwords, j0, j1, smask, k0, k1, dummy);
+            }
+        }
     }
     if (dirty) flush_rows(lds, 0, nbins, out);
 }
@@ -424,15 +454,23 @@ cr_status launch_an_mc(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, unsign
         return fail(CR_ERR_DEVICE, "occupancy query failed for the AN Monte Carlo kernel");
     const uint64_t RBlocks = ((1ull << n) + 64ull * RM - 1) / (64ull * RM);
     const uint64_t SC = (m + CJ - 1) / CJ;
-    const uint64_t items = RBlocks * SC;
-    uint64_t waves = (uint64_t)per_cu * cus;
+    const uint64_t resident = (uint64_t)per_cu * cus;
+    // row blocks per item: about 2^18 pairs per item (a power of two up to
+    // 4096), keeping at least 16 items per resident wave
+    const uint64_t chunk = m < CJ ? m : CJ;
+    uint32_t G = 1;
+    while (G < 4096 && (uint64_t)G * 64 * RM * chunk < (1ull << 18) &&
+           ((RBlocks + 2 * G - 1) / (2 * G)) * SC >= 16 * resident)
+        G *= 2;
+    const uint64_t items = ((RBlocks + G - 1) / G) * SC;
+    uint64_t waves = resident;
     if (waves > items) waves = items;
     CR_TRY(fill_zero_u64(counter, 1, stream));
     if (ev0) CR_HIP(hipEventRecord(ev0, stream));
-    uint64_t a_A = A, a_m = m, a_seed = seed, a_RB = RBlocks, a_SC = SC, a_items = items;
-    uint32_t a_n = n, a_nbins = nbins, a_rows = lds_rows;
+    uint64_t a_A = A, a_m = m, a_seed = seed, a_SC = SC, a_items = items;
+    uint32_t a_n = n, a_G = G, a_nbins = nbins, a_rows = lds_rows;
     unsigned long long *a_counter = counter, *a_out = out;
-    void *args[] = {&a_A, &a_n, &a_m, &a_seed, &a_RB, &a_SC, &a_items, &a_counter, &a_out, &a_nbins, &a_rows};
+    void *args[] = {&a_A, &a_n, &a_m, &a_seed, &a_G, &a_SC, &a_items, &a_counter, &a_out, &a_nbins, &a_rows};
     CR_HIP(hipLaunchKernel(k, dim3((uint32_t)waves), dim3(64), args, lds, stream));
     if (ev1) CR_HIP(hipEventRecord(ev1, stream));
     return CR_OK;

@@ -162,8 +162,9 @@ extern "C" cr_status cr_an_mc(cr_context *ctx, uint64_t A, uint32_t n, uint64_t 
     clear_error();
     if (!ctx || !hist) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (A == 0) return fail(CR_ERR_INVALID_ARGUMENT, "A must be >= 1");
-    if (n < 1 || n > 32) return fail(CR_ERR_UNSUPPORTED, "AN Monte Carlo supports 1 <= n <= 32 (got %u)", n);
-    if ((unsigned __int128)A << n >> 64) return fail(CR_ERR_UNSUPPORTED, "A << n exceeds 64 bits");
+    if (n < 1 || n > 63) return fail(CR_ERR_UNSUPPORTED, "AN Monte Carlo supports 1 <= n <= 63 (got %u)", n);
+    if ((unsigned __int128)A * ((1ull << n) - 1) >> 64)
+        return fail(CR_ERR_UNSUPPORTED, "codewords A*(2^n-1) exceed 64 bits");
     uint32_t bits = 0;
     for (unsigned __int128 x = (unsigned __int128)A * ((1ull << n) - 1); x; x >>= 1) ++bits;
     const uint32_t need = bits + 1;

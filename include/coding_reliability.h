@@ -65,6 +65,19 @@ cr_status cr_device_query(cr_context *ctx, cr_device_info *out);
  * counterpart in the reference, which allocated per call. */
 cr_status cr_device_memory(cr_context *ctx, uint64_t *free_bytes, uint64_t *total_bytes);
 
+/* Device ceilings the histogram kernels are measured against (no
+ * counterpart in the reference, which times its kernels only).  Runs one
+ * probe kernel on every resident single-wave workgroup for `iters` loop
+ * iterations and returns the rate it sustained:
+ *   probe 0: int32 VALU lane-ops/s of xor + popcount-accumulate (the inner
+ *            loop of the pair, grid and exhaustive Hamming kernels);
+ *   probe 1: Philox4x32-10 lane-ops/s, 60 per block (the Monte Carlo
+ *            generator: 20 64-bit multiplies and 40 xors per block);
+ *   probe 2: lane increments/s of wave-wide ds_add_u32 to lane-private LDS
+ *            counters in the histogram layout of every kernel.
+ * kernel_ms (may be NULL) receives the probe's device time. */
+cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t iters, double *units_per_s, double *kernel_ms);
+
 /* --- exhaustive AN ---------------------------------------------------------
  * raw[b] = #{ 0 <= i < j < num_words : popcount(A*i XOR A*j) == b } for
  * b < nbins, computed exactly on the GPU (uint64 counts).
@@ -192,10 +205,13 @@ cr_status cr_hamming_sdc_table(cr_context *ctx, uint32_t k, int method, uint64_t
 
 /* --- AN Monte Carlo -------------------------------------------------------------
  * hist[b] = #{(i, j) : i < 2^n, j < m, popcount(A*i XOR A*u_ij) == b} with u_ij
- * word j of the Philox stream (seed, i) masked to n bits (counter
- * (j/4, 0, i, 0x43520001), key (seed, seed >> 32), word j%4).  Replaces
+ * sample j of row i: Philox4x32-10 block b of counter
+ * (b, b >> 32, i mod 2^32, 0x43520001 ^ (i >> 32)), key (seed, seed >> 32);
+ * for n <= 32, b = j/4 and u_ij = word j%4; for n > 32, b = j/2 and u_ij =
+ * word 2(j%2) | word 2(j%2)+1 << 32; masked to n bits.  Replaces
  * dancoding_mc_shared's cuRAND Sobol draw (an_coding_mc.cu:94-158), which is
- * not reproducible; n <= 32. */
+ * not reproducible; 1 <= n <= 63 with A*(2^n - 1) < 2^64 (the reference's
+ * ccmax <= 64, an_coding_mc.cu:163-188). */
 cr_status cr_an_mc(cr_context *ctx, uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint64_t *hist, uint32_t nbins,
                    double *kernel_ms);
 

@@ -512,31 +512,54 @@ void or_hamming_mc(unsigned k, uint64_t seed, uint32_t stream, uint64_t s_begin,
 }
 
 /*
+ * Sample j of row i of the AN Monte Carlo (this library's definition; the
+ * reference draws (T)(2^n * U) from closed cuRAND streams, an_coding_mc.cu:125):
+ * Philox block b of counter (b, b >> 32, i mod 2^32, 0x43520001 ^ (i >> 32)),
+ * key (seed, seed >> 32); n <= 32: b = j/4, word j%4; n > 32: b = j/2, the
+ * 64-bit word out[2(j%2)] | out[2(j%2)+1] << 32; masked to n bits.
+ */
+uint64_t or_an_mc_sample(uint64_t seed, uint64_t i, uint64_t j, unsigned n)
+{
+    const unsigned per = n <= 32 ? 4 : 2;
+    const uint64_t blk = j / per;
+    const uint32_t ctr[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)i, 0x43520001u ^ (uint32_t)(i >> 32)};
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t o[4];
+    or_philox4x32_10(ctr, key, o);
+    const uint64_t dm = n >= 64 ? ~0ull : ((1ull << n) - 1);
+    if (per == 4) return (uint64_t)o[j % 4] & dm;
+    const unsigned h = (unsigned)(j % 2);
+    return ((uint64_t)o[2 * h] | ((uint64_t)o[2 * h + 1] << 32)) & dm;
+}
+
+/*
  * Monte Carlo AN histogram (replaces dancoding_mc_shared,
- * an_coding_mc.cu:94-158): every word i in [0,2^n) is compared with m
- * codewords A*u, u = or_mc_word(seed, stream = i & 0xFFFFFFFF, s = (i>>32)*m + j, n) for j<m.
+ * an_coding_mc.cu:94-158): every word i in [i_begin, i_end) of [0, 2^n) is
+ * compared with the m codewords A*u_ij, u_ij = or_an_mc_sample(seed, i, j, n).
  * hist is the raw (word, sample) ordered-pair histogram.
  */
-void or_an_mc(uint64_t A, unsigned n, uint64_t m, uint64_t seed, uint64_t *hist, unsigned nbins)
+void or_an_mc_rows(uint64_t A, unsigned n, uint64_t m, uint64_t seed, uint64_t i_begin, uint64_t i_end,
+                   uint64_t *hist, unsigned nbins)
 {
-    const uint64_t M = 1ull << n;
 #pragma omp parallel
     {
         uint64_t local[OR_MAXBINS];
         memset(local, 0, sizeof(local));
 #pragma omp for schedule(static)
-        for (uint64_t i = 0; i < M; ++i) {
+        for (uint64_t i = i_begin; i < i_end; ++i) {
             const uint64_t w = A * i;
-            for (uint64_t j = 0; j < m; ++j) {
-                const uint64_t u = or_mc_word(seed, (uint32_t)i, (i >> 32) * m + j, n);
-                ++local[popc64(w ^ (A * u))];
-            }
+            for (uint64_t j = 0; j < m; ++j) ++local[popc64(w ^ (A * or_an_mc_sample(seed, i, j, n)))];
         }
         for (unsigned b = 0; b < nbins; ++b) {
 #pragma omp atomic
             hist[b] += local[b];
         }
     }
+}
+
+void or_an_mc(uint64_t A, unsigned n, uint64_t m, uint64_t seed, uint64_t *hist, unsigned nbins)
+{
+    or_an_mc_rows(A, n, m, seed, 0, 1ull << n, hist, nbins);
 }
 
 /* ------------------------------------------------------------------------ */

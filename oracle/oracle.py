@@ -23,9 +23,11 @@ Pinning of each oracle function (what checks it, see ``tests/test_oracle.py``):
   dual code (all k, including the (72,64) code the reference cannot run).
 * ``philox4x32_10``: known-answer vectors produced from the open-source
   rocRAND header (tests/golden/make_philox_vectors.py).
-* ``mc_word`` / ``hamming_mc`` / ``an_mc``: parity unpinned against the
-  reference (it uses closed cuRAND Sobol32 streams); pinned statistically
-  (4-sigma binomial bounds vs exhaustive / MacWilliams truth).
+* ``mc_word`` / ``hamming_mc`` / ``an_mc`` / ``an_mc_sample``: parity
+  unpinned against the reference (it uses closed cuRAND Sobol32 streams);
+  pinned statistically (4-sigma binomial bounds vs exhaustive / MacWilliams
+  truth, the binomial closed form of A = 1), and the sample words against
+  the Philox known-answer vectors through their counter layout.
 * ``an_candidates``: independent NAF-weight recurrence in pure Python
   (``naf_weight``) brute-forced for small (k, |A|).
 """
@@ -83,6 +85,10 @@ def lib():
         L.or_mc_word.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint]
         L.or_hamming_mc.argtypes = [C.c_uint, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, u64p, C.c_uint]
         L.or_an_mc.argtypes = [C.c_uint64, C.c_uint, C.c_uint64, C.c_uint64, u64p, C.c_uint]
+        L.or_an_mc_rows.argtypes = [C.c_uint64, C.c_uint, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64, u64p,
+                                    C.c_uint]
+        L.or_an_mc_sample.restype = C.c_uint64
+        L.or_an_mc_sample.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint]
         L.or_an_min_naf.restype = C.c_uint64
         L.or_an_min_naf.argtypes = [C.c_uint64, C.c_uint]
         L.or_an_candidates.restype = C.c_uint64
@@ -293,6 +299,17 @@ def an_mc(A: int, n: int, m: int, seed: int, nbins: int = 65) -> list[int]:
     h = np.zeros(nbins, dtype=np.uint64)
     lib().or_an_mc(A, n, m, seed, _p(h), nbins)
     return [int(x) for x in h]
+
+
+def an_mc_rows(A: int, n: int, m: int, seed: int, i0: int, i1: int, nbins: int = 65) -> list[int]:
+    """The AN Monte Carlo histogram restricted to the rows i in [i0, i1)."""
+    h = np.zeros(nbins, dtype=np.uint64)
+    lib().or_an_mc_rows(A, n, m, seed, i0, i1, _p(h), nbins)
+    return [int(x) for x in h]
+
+
+def an_mc_sample(seed: int, i: int, j: int, n: int) -> int:
+    return int(lib().or_an_mc_sample(seed, i, j, n))
 
 
 # --- ANcandidates -----------------------------------------------------------------

@@ -131,6 +131,33 @@ def test_an_mc_bitexact_vs_oracle(ctx, A, n, m):
     assert h == O.an_mc(A, n, m, 1234)[: len(h)]
 
 
+@pytest.mark.parametrize("A", [61, 3])
+def test_an_mc_64bit_samples_bitexact_vs_oracle(ctx, A):
+    # n = 33: 64-bit samples (two Philox words each, the row's high bits in
+    # the counter's fourth word) and 64-bit codewords, every one of the 2^33
+    # rows against the oracle's restatement of the same sample definition
+    n, m = 33, 2
+    h, ms = ctx.an_mc(A, n, m, seed=4321)
+    print(f"AN MC A={A} n={n} m={m}: {ms:.1f} ms", flush=True)
+    assert sum(h) == (1 << n) * m
+    assert h == O.an_mc(A, n, m, 4321)[: len(h)]
+
+
+@pytest.mark.parametrize("A,n,m", [(1, 40, 4), (1024, 40, 2), (1, 36, 64)])
+def test_an_mc_64bit_binomial_closed_form(ctx, A, n, m):
+    # for A a power of two the distance of A*i and A*u is popcount(i ^ u), and
+    # u uniform on n bits makes it Binomial(n, 1/2) for every row: each bin is
+    # within 4 sigma of 2^n m C(n, d) / 2^n
+    h, ms = ctx.an_mc(A, n, m, seed=2026)
+    print(f"AN MC A={A} n={n} m={m}: {ms:.1f} ms ({(1 << n) * m / ms / 1e9:.2f} Tpairs/s)", flush=True)
+    tot = (1 << n) * m
+    assert sum(h) == tot
+    for d in range(n + 1):
+        p = math.comb(n, d) / 2.0 ** n
+        assert abs(h[d] - tot * p) <= 4 * math.sqrt(tot * p * (1 - p)) + 1, d
+    assert not any(h[n + 1:])
+
+
 def test_an_mc_4sigma_vs_exhaustive(ctx):
     A, n, m = 58659, 16, 4096
     h, _ = ctx.an_mc(A, n, m, seed=77)
@@ -148,4 +175,6 @@ def test_invalid(ctx):
     with pytest.raises(cra.CRError):
         ctx.hamming_weights(32, cra.HAM_MC, first=1, count=10)
     with pytest.raises(cra.CRError):
-        ctx.an_mc(61, 33, 10, 1)
+        ctx.an_mc(61, 60, 10, 1)  # codewords beyond 64 bits
+    with pytest.raises(cra.CRError):
+        ctx.an_mc(1, 64, 10, 1)

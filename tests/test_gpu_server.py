@@ -176,3 +176,75 @@ def test_burst_of_small_requests(ctx):
         ref = ctx.an_pairs(A, 1 << k)[0]
         assert g == ref[: len(g)], (A, k)
     assert st["completed"] == len(jobs)
+
+
+def test_c5_served_mix_at_stated_sizes(ctx):
+    """The served configuration at its stated sizes, with seeded Poisson
+    arrivals: exhaustive AN requests (k uniform in 18..24, odd 16-bit A),
+    integer-grid AN requests (k = 32, stride 2^12 + 1 = 4097: the words
+    A*(4097 i) for 4097 i < 2^32, i.e. 1048321 words, every unordered pair)
+    and Monte Carlo (72,64) Extended Hamming requests of 2^30 samples each.
+    Every result equals the one-shot entry point's (cr_an_pairs, which takes
+    the spectral path where it is cheaper; cr_hamming_weights)."""
+    import numpy as np
+    g = np.random.Generator(np.random.Philox(20261203))
+    kinds = [0, 1, 2] * 4
+    g.shuffle(kinds)
+    jobs, arrivals, t = [], [], 0.0
+    for i, kind in enumerate(kinds):
+        A = int(2 * g.integers(1 << 14, 1 << 15) + 1)
+        if kind == 0:
+            jobs.append(dict(kind=cra.JOB_AN_EXHAUSTIVE, k=int(g.integers(18, 25)), A=A))
+        elif kind == 1:
+            jobs.append(dict(kind=cra.JOB_AN_GRID, k=32, A=A, stride=4097))
+        else:
+            jobs.append(dict(kind=cra.JOB_HAMMING_MC, k=64, samples=1 << 30, seed=20261203, stream=i))
+        t += float(g.exponential(1 / 4.0))  # 4 requests per second
+        arrivals.append(t)
+    print("C5 jobs:", jobs, flush=True)
+    with cra.Server(ctx, iteration_ms=20) as srv:
+        t0 = srv.now()
+        ids = []
+        for j, at in zip(jobs, arrivals):
+            delay = at - (srv.now() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            ids.append(srv.submit(**j))
+        for rid in ids:
+            srv.wait(rid, timeout_ms=120000)
+        got = [srv.result(rid) for rid in ids]
+        st = srv.stats()
+    assert st["completed"] == len(jobs)
+    lat = sorted(info["finish_s"] - info["submit_s"] for _, info in got)
+    print(f"C5 served: p50 {lat[len(lat) // 2]:.3f} s, max {lat[-1]:.3f} s, {st['iterations']} iterations",
+          flush=True)
+    for j, (h, info) in zip(jobs, got):
+        if j["kind"] == cra.JOB_AN_EXHAUSTIVE:
+            want = ctx.an_pairs_auto(j["A"], 1 << j["k"])[0]
+            N = 1 << j["k"]
+            assert info["units"] == N * (N - 1) // 2
+        elif j["kind"] == cra.JOB_AN_GRID:
+            m = ((1 << 32) + 4096) // 4097
+            assert m == 1048321
+            want = ctx.an_pairs(j["A"] * 4097, m)[0]
+            assert sum(h) == m * (m - 1) // 2 and info["units"] == m * (m - 1) // 2
+        else:
+            want = direct(ctx, j)
+            assert sum(h) == 1 << 30
+        assert h == want[: len(h)] and not any(want[len(h):]), j
+
+
+@pytest.mark.parametrize("A", [16385, 58659])
+def test_k32_stride_grid_request_vs_oracle(ctx, A):
+    # the C5 grid request's code path (k = 32, 16-bit A, multiplier A*stride
+    # above 2^22: the generic two-dword pair mode) at a stride the oracle
+    # enumerates quickly: 4097*64 = 262208 gives 16381 words
+    s = 4097 * 64
+    m = ((1 << 32) + s - 1) // s
+    with cra.Server(ctx, iteration_ms=5) as srv:
+        rid = srv.submit(cra.JOB_AN_GRID, 32, A=A, stride=s)
+        srv.wait(rid, timeout_ms=60000)
+        h, info = srv.result(rid)
+    want = O.an_pairs_raw(A * s, m, len(h))
+    assert h == want
+    assert info["units"] == m * (m - 1) // 2

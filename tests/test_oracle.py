@@ -264,3 +264,31 @@ def test_reported_form_from_macwilliams_equals_sphere_enumeration():
     for k in (4, 8):
         for b in (False, True):
             assert hamming_reported_from_ext(k, macwilliams_weights(k), b) == O.hamming_bruteforce(k, b)
+
+
+def test_an_mc_sample_counter_layout():
+    # the AN Monte Carlo sample words through the Philox generator (pinned by
+    # its known-answer vectors): counter (b, b >> 32, i mod 2^32,
+    # 0x43520001 ^ (i >> 32)), key (seed, seed >> 32); 4 words per block for
+    # n <= 32, 2 (low, high) for n > 32; masked to n bits
+    seed = 0x123456789ABCDEF0
+    key = [seed & 0xFFFFFFFF, seed >> 32]
+    for i, j, n in [(5, 7, 16), (77, 0, 32), (3, 9, 33), ((1 << 33) + 5, 6, 40), ((1 << 48) - 1, 3, 48)]:
+        per = 4 if n <= 32 else 2
+        b = j // per
+        o = O.philox4x32_10([b & 0xFFFFFFFF, b >> 32, i & 0xFFFFFFFF, 0x43520001 ^ (i >> 32)], key)
+        w = o[j % 4] if per == 4 else o[2 * (j % 2)] | (o[2 * (j % 2) + 1] << 32)
+        assert O.an_mc_sample(seed, i, j, n) == w & ((1 << n) - 1), (i, j, n)
+
+
+@pytest.mark.parametrize("n,i0", [(20, 0), (36, 1 << 35), (47, (1 << 46) + 12345)])
+def test_an_mc_samples_uniform(n, i0):
+    # A = 1: the distance of row i to its samples is popcount(i ^ u); uniform
+    # samples make it Binomial(n, 1/2) (4 sigma per bin over 2^14 rows x 256)
+    rows, m = 1 << 14, 256
+    h = O.an_mc_rows(1, n, m, 99, i0, i0 + rows)
+    tot = rows * m
+    assert sum(h) == tot
+    for d in range(n + 1):
+        p = math.comb(n, d) / 2.0 ** n
+        assert abs(h[d] - tot * p) <= 4 * math.sqrt(tot * p * (1 - p)) + 1, d

@@ -287,3 +287,18 @@ def test_cli_hamming_cpu_entry_point_n40(tmp_path):
     want = hamming_reported_from_ext(n, macwilliams_weights(n, masks32), True)
     run_cli(tmp_path, "-e", "1", "-c", "-n", str(n), "-b", "-f")
     _check_wide_counts(tmp_path / f"hamming_cpu_n{n}_h7.csv", want)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+def test_cli_an_mc_beyond_32_bits(tmp_path):
+    # run_ancoding_mc at n = 40 (an_coding_mc.cu:163-188 dispatches up to
+    # ccmax 64): 2^40 words x 2 Philox samples of 40 bits; the reported counts
+    # are (2^40 / 2) x the raw histogram, summing to exactly 4^40
+    run_cli(tmp_path, "-a", "61", "-n", "40", "-m", "2", "-f")
+    rows, summary = _text(tmp_path / "ancoding_mc_gpu_m2_A61_n40_h6.csv")
+    assert len(rows) == 47
+    assert summary[0] == '"Total",' + f"{float(4 ** 40):.6g}"
+    counts = [float(r.split(",")[1]) for r in rows]
+    assert counts[1] == 0  # no two multiples of 61 differ in one bit
+    assert abs(sum(counts) / 4 ** 40 - 1) < 1e-9
