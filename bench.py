@@ -170,49 +170,68 @@ def run_batched(ctx, autotune: bool = True) -> dict:
     return out
 
 
-# Work per unit of the other kernels, in three classes, each measured against
-# its own ceiling (cr_probe_ceiling, run in the same process):
-#   valu    int32 lane-ops on the VALU, the histogram increment counted as one
-#           (it costs the LDS address computation), as the pair kernel's
-#           2W+1 does;
-#   philox  Philox4x32-10 lane-ops, 60 per block: 15 per 32-bit word drawn
-#           (4 words per block), 30 per 64-bit word;
-#   lds     histogram increments (one wave-wide ds_add_u32 lane each).
-# Per unit: Hamming weight of an enumerated word: xor, popcount-accumulate,
-# increment (the syndrome and popcount split into per-lane and per-iteration
-# parts over an aligned block), 3 / 0 / 1.  A sampled 32-bit word: 4 byte
-# extracts for the (linear) syndrome's per-byte tables, 3 xors, 2 popcounts,
-# 9 / 0 / 1 (64-bit: 16 / 0 / 1); Monte Carlo adds the generator, 15 or 30.
-# Reference grid pair: xor + popcount + increment, 3 / 0 / 1 (a row block's
-# common high dword folds into the popcount).  AN Monte Carlo pair: 2W + 1
-# and the multiply of the sample by A, 4 / 15 / 1.  ANcandidates step (A, d):
-# c += A, 3c, xor and popcount on 64 bits, a min: 9 / 0 / 0.
-# The VALU fraction adds the two VALU classes' times (valu/P_valu +
-# philox/P_philox per unit); the reported frac is the larger of it and the
-# LDS fraction (the two pipelines run concurrently), named by "bound".
-def _kernel_entry(units: float, ms: float, work: tuple, unit: str, ceil: dict, workload: str) -> dict:
-    valu, philox, lds = work
+# The other kernels are measured against the same two ceilings as the pair
+# kernel: the VALU issue ceiling and the LDS increment ceiling, both measured
+# in this run (measure_ceilings).  Per unit of work (word, pair, step) each
+# kernel executes `valu` VALU instructions -- the SQ_INSTS_VALU counts of its
+# profile under profiles/, divided by the units of the launch -- and `lds`
+# histogram increments (wave-wide ds_add_u32 lanes).  frac_valu is the VALU
+# issue utilisation (executed instructions/s over the ceiling), frac_lds the
+# LDS increment rate over its ceiling; frac is the larger (the two pipelines
+# run concurrently), named by "bound".  `ops` is the algorithmic int32 work
+# per unit for reference: xor + popcount + increment per enumerated word or
+# pair (2W+1 with the high dword folded), 4 byte lookups + 3 xors + 2
+# popcounts per sampled 32-bit word (8 + 7 + 2 for 64-bit), 60 per Philox
+# block (15 per 32-bit Monte Carlo word, 30 per 64-bit), 9 per ANcandidates
+# step.  The sampled Hamming kernels also read syndrome tables in LDS (4 or 8
+# ds_read_u8 per word), not counted in frac_lds.
+KERNEL_WORK = {
+    # name: (executed VALU per unit, LDS increments per unit, algorithmic ops per unit, profile)
+    "hamming_exhaustive_39_32": (3.09, 1, 3, "profiles/ham_kernel_39_32_exhaustive_grid_mc_and_72_64_mc.md"),
+    "hamming_grid_39_32": (None, 1, 9, "latency-bound launch"),
+    "hamming_mc_39_32": (26.53, 1, 24, "profiles/ham_kernel_39_32_exhaustive_grid_mc_and_72_64_mc.md"),
+    "hamming_mc_72_64": (51.27, 1, 46, "profiles/ham_kernel_39_32_exhaustive_grid_mc_and_72_64_mc.md"),
+    "an_grid1d_ref": (3.01, 1, 3, "profiles/rect_kernel_grid1d_A61_k32_m1001.md"),
+    "an_mc": (18.04, 1, 19, "profiles/an_mc_kernel_A61_k24_m65536.md"),
+    "an_candidates": (None, 0, 9, "lower bound: algorithmic steps of the maximising A"),
+}
+
+
+def _kernel_entry(name: str, units: float, ms: float, unit: str, ceil: dict, workload: str) -> dict:
+    valu, lds, ops, src = KERNEL_WORK[name]
     rate = units / (ms / 1e3)
-    t_valu = valu / ceil["valu"] + philox / ceil["philox"]
-    f_valu = rate * t_valu
+    f_valu = rate * (valu if valu is not None else ops) / ceil["valu"]
     f_lds = rate * lds / ceil["lds"] if lds else 0.0
     bound = "valu" if f_valu >= f_lds else "lds_atomic"
     return {"workload": workload, "kernel_ms": round(ms, 3), "rate": round(rate / 1e9, 3), "rate_unit": unit,
-            "work_per_unit": {"valu": valu, "philox": philox, "lds_increments": lds},
-            "achieved_tops": round(rate * (valu + philox) / 1e12, 3),
+            "valu_per_unit": valu if valu is not None else ops, "valu_source": src if valu is not None else
+            "algorithmic ops (" + src + ")", "lds_increments_per_unit": lds, "ops_per_unit": ops,
+            "achieved_tops": round(rate * ops / 1e12, 3),
             "frac_valu": round(f_valu, 4), "frac_lds": round(f_lds, 4), "bound": bound,
             "frac": round(max(f_valu, f_lds), 4)}
 
 
+def _ceilings_json(ceil: dict) -> dict:
+    return {"valu": round(ceil["valu"] / 1e12, 4), "lds": round(ceil["lds"] / 1e12, 4),
+            "unit": "T/s: VALU lane-instructions, LDS lane increments (cr_probe_ceiling, this run)",
+            "probes": {kk: round(v / 1e12, 4) for kk, v in ceil["probes"].items()}}
+
+
 def measure_ceilings(ctx) -> dict:
-    """The device ceilings of this run (cr_probe_ceiling): best of three
-    launches of each probe after one untimed launch."""
-    iters = {0: 1 << 14, 1: 1 << 11, 2: 1 << 14}
-    out = {}
-    for key, probe in (("valu", 0), ("philox", 1), ("lds", 2)):
+    """The device ceilings of this run (cr_probe_ceiling), each the best of
+    three launches after one untimed one: VALU issue (the best of the
+    xor+popcount, Philox and mixed probes, in lane-instructions/s) and LDS
+    increments (lane increments/s, the better of the two lane patterns)."""
+    iters = {0: 1 << 15, 1: 1 << 12, 2: 1 << 14, 3: 1 << 12, 4: 1 << 14, 5: 1 << 14}
+
+    def best(probe):
         ctx.probe_ceiling(probe, iters[probe])
-        out[key] = max(ctx.probe_ceiling(probe, iters[probe])[0] for _ in range(3))
-    return out
+        return max(ctx.probe_ceiling(probe, iters[probe])[0] for _ in range(3))
+
+    r = {p: best(p) for p in range(6)}
+    return {"valu": max(r[0], r[1], r[3], r[4]), "lds": max(r[2], r[5]),
+            "probes": {"valu_xor_popcount_8": r[0], "valu_philox": r[1], "lds_scattered_rows": r[2],
+                       "valu_philox_pair_mix": r[3], "valu_xor_popcount_16": r[4], "lds_same_row": r[5]}}
 
 
 def _warm(f):
@@ -231,11 +250,12 @@ def run_kernels(ctx, ceil: dict) -> dict:
     k = 32
     h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_RANGE, 0, 1 << k))
     assert sum(h) == 1 << k
-    out["hamming_exhaustive_39_32"] = _kernel_entry(1 << k, ms, (3, 0, 1), "Gwords/s", ceil,
+    out["hamming_exhaustive_39_32"] = _kernel_entry("hamming_exhaustive_39_32", 1 << k, ms, "Gwords/s", ceil,
                                                     "(39,32) weights of all 2^32 data words")
     h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 20, stride=65537))
     assert sum(h) == 1 << 20
-    e = _kernel_entry(1 << 20, ms, (9, 0, 1), "Gwords/s", ceil, "(39,32) integer grid d = i*65537 mod 2^32, i < 2^20")
+    e = _kernel_entry("hamming_grid_39_32", 1 << 20, ms, "Gwords/s", ceil,
+                      "(39,32) integer grid d = i*65537 mod 2^32, i < 2^20")
     # 2^20 words are well under a microsecond of device work: the launch is
     # latency-bound, so its floor (the same kernel over 4096 words) is given
     _, floor_ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 12, stride=65537))
@@ -244,26 +264,26 @@ def run_kernels(ctx, ceil: dict) -> dict:
     S = 1 << 34
     h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_MC, 0, S, seed=2026, stream=0))
     assert sum(h) == S
-    out["hamming_mc_39_32"] = _kernel_entry(S, ms, (9, 15, 1), "Gwords/s", ceil, "(39,32) 2^34 Philox words")
+    out["hamming_mc_39_32"] = _kernel_entry("hamming_mc_39_32", S, ms, "Gwords/s", ceil, "(39,32) 2^34 Philox words")
     S = 1 << 30
     h, ms = _warm(lambda: ctx.hamming_weights(64, cra.HAM_MC, 0, S, seed=2026, stream=1))
     assert sum(h) == S
-    out["hamming_mc_72_64"] = _kernel_entry(S, ms, (16, 30, 1), "Gwords/s", ceil, "(72,64) 2^30 Philox words")
+    out["hamming_mc_72_64"] = _kernel_entry("hamming_mc_72_64", S, ms, "Gwords/s", ceil, "(72,64) 2^30 Philox words")
     h, ms = _warm(lambda: ctx.an_grid_ref(1, 32, 1001, 0, 61))
     pairs = sum(h)
-    out["an_grid1d_ref"] = _kernel_entry(pairs, ms, (3, 0, 1), "Gpairs/s", ceil,
+    out["an_grid1d_ref"] = _kernel_entry("an_grid1d_ref", pairs, ms, "Gpairs/s", ceil,
                                          f"reference grid 1D, A=61, k=32, m=1001 (38-bit codewords, {pairs} pairs)")
     m = 1 << 16
     h, ms = _warm(lambda: ctx.an_mc(61, 24, m, 16384))
     assert sum(h) == (1 << 24) * m
-    out["an_mc"] = _kernel_entry((1 << 24) * m, ms, (4, 15, 1), "Gpairs/s", ceil,
+    out["an_mc"] = _kernel_entry("an_mc", (1 << 24) * m, ms, "Gpairs/s", ceil,
                                  "AN Monte Carlo, A=61, k=24, 2^16 samples")
     best, cands, ms = _warm(lambda: ctx.an_candidates(24, 16))
     # pruning skips most (A, d) steps, so the work is counted as the steps the
     # candidates (A attaining the maximum, never pruned) must run in full: a
     # lower bound of the executed steps, hence a lower bound of the fraction
     steps = len(cands) * ((1 << 24) - 1)
-    e = _kernel_entry(steps, ms, (9, 0, 0), "Gsteps/s", ceil,
+    e = _kernel_entry("an_candidates", steps, ms, "Gsteps/s", ceil,
                       "ANcandidates cell k=24, |A|=16 (16384 odd A); rate = steps of the A attaining the maximum "
                       "(each runs all 2^24-1 steps) per second, a lower bound")
     e["best"], e["candidates"] = best, len(cands)
@@ -556,9 +576,9 @@ def main() -> int:
     log(f"device {info['name']} {info['arch']} CUs={info['compute_units']} clock={info['max_clock_mhz']} MHz")
     tune = dict(rows_per_lane=args.rows_per_lane, chunk_blocks=args.chunk_blocks)
     ceil = measure_ceilings(ctx)
-    log("ceilings: " + ", ".join(f"{kk} {v / 1e12:.2f} T/s" for kk, v in ceil.items()))
+    log("ceilings: " + ", ".join(f"{kk} {v / 1e12:.3f} T/s" for kk, v in ceil["probes"].items()))
     if args.ceilings_only:
-        print(json.dumps({"ceilings": {kk: round(v / 1e12, 3) for kk, v in ceil.items()}}), flush=True)
+        print(json.dumps({"ceilings": _ceilings_json(ceil)}), flush=True)
         ctx.close()
         return 0
     if args.kernels_only:
@@ -634,7 +654,8 @@ def main() -> int:
             "frac": round(pair_rate * ops_per_pair / ceil["valu"], 4),
             "ops_per_pair": ops_per_pair,
             "ops_per_pair_2W_plus_1": 2 * W + 1,
-            "peak_source": "cr_probe_ceiling(0) in this run: xor + popcount-accumulate on 8 independent chains",
+            "peak_source": "cr_probe_ceiling in this run: the best VALU issue rate of the xor+popcount, Philox "
+                           "and mixed probes",
             "nominal_peak": round(nominal, 3),
             "nominal_frac": round(pair_rate * ops_per_pair / 1e12 / nominal, 4),
             "nominal_note": f"{cus} CU x 64 lanes x {PEAK_CLOCK_GHZ} GHz, one VALU instruction per CU per clock",
@@ -690,8 +711,7 @@ def main() -> int:
                                f"all C(2^{k},2) pairs, {width_bits + 1} bins",
                    "A": A, "k": k, "pairs_per_step": pairs_per_step, "parallelism": "single GPU"},
         "roofline": roofline,
-        "ceilings": {kk: round(v / 1e12, 4) for kk, v in ceil.items()},
-        "ceilings_unit": "T/s: valu and philox int32 lane-ops, lds lane increments (cr_probe_ceiling)",
+        "ceilings": _ceilings_json(ceil),
         "spectral": spectral,
         "cpu_baseline": cpu,
         "batched": batched,

@@ -1,17 +1,21 @@
 // Issue-rate probes: the device ceilings the histogram kernels are bound by,
-// measured on the device they run on (gfx950, wave64).  Each probe runs the
-// instruction mix of one kind of kernel and nothing else, on every resident
-// single-wave workgroup, for a fixed number of iterations:
-//   kProbeValu   xor + popcount-accumulate (v_xor_b32, v_bcnt_u32_b32) on 8
-//                independent chains: the pair, grid and exhaustive Hamming
-//                inner loops; 2 lane-ops per chain step
-//   kProbePhilox Philox4x32-10 blocks (4 independent streams per lane): the
-//                Monte Carlo kernels' generator; 60 lane-ops per block
-//                (10 rounds of 2 multiplies, 2 high halves and 2 xors), the
-//                count the rooflines use for it
-//   kProbeLds    wave-wide ds_add_u32 to lane-private LDS counters (row
-//                stride 64 dwords, the histogram layout of every kernel):
-//                the per-pair histogram increment; 1 unit per lane increment
+// measured on the device they run on (gfx950, wave64).  Each probe runs one
+// instruction mix and nothing else, on every resident single-wave
+// workgroup, for a fixed number of iterations; the unit is a VALU
+// lane-instruction (the loop bodies were checked in the gfx950 assembly) or
+// an LDS lane increment:
+//   0 kProbeValu    xor + popcount-accumulate (v_xor_b32, v_bcnt_u32_b32) on
+//                   8 independent chains: the pair, grid and exhaustive
+//                   Hamming inner loops; 16 per iteration
+//   1 kProbePhilox  Philox4x32-10 blocks on 4 independent streams per lane
+//                   (20 v_mad_u64_u32 + 40 v_xor_b32 per block)
+//   2 kProbeLds     wave-wide ds_add_u32 to lane-private LDS counters in the
+//                   histogram layout of every kernel (row stride 64 dwords),
+//                   the lanes of an instruction in different rows
+//   3 kProbeMixed   Philox blocks whose words go through the AN Monte Carlo
+//                   pair work (multiply, xor, popcount): 72 per block
+//   4 kProbeValu16  as 0 on 16 chains
+//   5 kProbeLdsRow  as 2, all lanes of an instruction in one row
 #include "cr_device.h"
 #include "cr_internal.h"
 
@@ -20,7 +24,8 @@ namespace {
 
 using namespace dev;
 
-constexpr uint32_t kProbeValu = 0, kProbePhilox = 1, kProbeLds = 2;
+constexpr uint32_t kProbeValu = 0, kProbePhilox = 1, kProbeLds = 2, kProbeMixed = 3, kProbeValu16 = 4,
+                   kProbeLdsRow = 5, kProbeCount = 6;
 constexpr int kValuChains = 8;
 constexpr int kLdsRows = 16;
 
@@ -67,22 +72,90 @@ __global__ __launch_bounds__(64) void probe_philox(uint32_t iters, uint32_t seed
     out[blockIdx.x * 64 + lane] = acc;
 }
 
+// The LDS probes keep kLdsRows rows of 64 counters (4 KB per wave, so the
+// resident waves per CU are not limited by LDS); lane l owns column l of
+// every row.  Scattered: the lanes of one instruction hit different rows.
 __global__ __launch_bounds__(64) void probe_lds(uint32_t iters, uint32_t seed, uint32_t *out) {
-    __shared__ uint32_t lds[64 * kStride];
+    __shared__ uint32_t lds[kLdsRows * kStride];
     const uint32_t lane = threadIdx.x;
-    lds_clear(lds, 64);
-    // kLdsRows different rows per lane (fixed over the loop), each in the
-    // lane's own bank
+    lds_clear(lds, kLdsRows);
     uint32_t *p[kLdsRows];
 #pragma unroll
-    for (int j = 0; j < kLdsRows; ++j) p[j] = lds + ((lane * 7 + j * 5 + seed) & 63) * kStride + lane;
+    for (int j = 0; j < kLdsRows; ++j) p[j] = lds + ((lane * 7 + j * 5 + seed) % kLdsRows) * kStride + lane;
     for (uint32_t i = 0; i < iters; ++i) {
 #pragma unroll
         for (int j = 0; j < kLdsRows; ++j) lds_inc(p[j]);
     }
     __syncthreads();
     uint32_t t = 0;
-    for (uint32_t r = 0; r < 64; ++r) t += lds[r * kStride + lane];
+    for (uint32_t r = 0; r < kLdsRows; ++r) t += lds[r * kStride + lane];
+    out[blockIdx.x * 64 + lane] = t;
+}
+
+// Philox blocks whose words each go through the AN Monte Carlo pair work
+// (24-bit multiply, xor, popcount-accumulate): the mix of the Monte Carlo
+// kernels without their LDS increments; 60 + 4 x 3 VALU per block.
+__global__ __launch_bounds__(64) void probe_mixed(uint32_t iters, uint32_t seed, uint32_t *out) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t c[2][4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+        c[s][0] = lane + 64 * s;
+        c[s][1] = blockIdx.x;
+        c[s][2] = seed;
+        c[s][3] = kMcTag;
+    }
+    uint32_t w = lane * 0x01000193u + blockIdx.x, acc = 0;
+    for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            philox4x32_10(c[s], seed, seed ^ 0x5bd1e995u);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc = __popc(w ^ ((c[s][u] & 0x00FFFFFFu) * 61u)) + acc;
+        }
+    }
+    out[blockIdx.x * 64 + lane] = acc ^ c[0][0] ^ c[1][1];
+}
+
+// xor + popcount-accumulate on 16 independent chains.
+__global__ __launch_bounds__(64) void probe_valu16(uint32_t iters, uint32_t seed, uint32_t *out) {
+    const uint32_t lane = threadIdx.x;
+    uint32_t x[16], c[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        x[j] = (lane + 1) * 0x9E3779B9u ^ (seed + j * 0x85EBCA6Bu) ^ blockIdx.x;
+        c[j] = 0;
+    }
+    for (uint32_t i = 0; i < iters; ++i) {
+        const uint32_t s = i * 0x01000193u + seed;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            x[j] ^= s;
+            c[j] = __popc(x[j]) + c[j];
+        }
+    }
+    uint32_t t = 0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) t += c[j] ^ x[j];
+    out[blockIdx.x * 64 + lane] = t;
+}
+
+// LDS increments with every lane of an instruction in the same row (the
+// 64 consecutive dwords of one bin), the row changing per instruction.
+__global__ __launch_bounds__(64) void probe_lds_row(uint32_t iters, uint32_t seed, uint32_t *out) {
+    __shared__ uint32_t lds[kLdsRows * kStride];
+    const uint32_t lane = threadIdx.x;
+    lds_clear(lds, kLdsRows);
+    uint32_t *p[kLdsRows];
+#pragma unroll
+    for (int j = 0; j < kLdsRows; ++j) p[j] = lds + ((j * 5 + seed) % kLdsRows) * kStride + lane;
+    for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int j = 0; j < kLdsRows; ++j) lds_inc(p[j]);
+    }
+    __syncthreads();
+    uint32_t t = 0;
+    for (uint32_t r = 0; r < kLdsRows; ++r) t += lds[r * kStride + lane];
     out[blockIdx.x * 64 + lane] = t;
 }
 
@@ -95,12 +168,14 @@ extern "C" cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t 
                                       double *kernel_ms) {
     clear_error();
     if (!ctx || !units_per_s) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (probe > kProbeLds) return fail(CR_ERR_INVALID_ARGUMENT, "unknown probe %u", probe);
+    if (probe >= kProbeCount) return fail(CR_ERR_INVALID_ARGUMENT, "unknown probe %u", probe);
     if (iters == 0 || iters > (1u << 24)) return fail(CR_ERR_INVALID_ARGUMENT, "iters=%u outside [1, 2^24]", iters);
     CR_HIP(hipSetDevice(ctx->device));
-    const void *k = probe == kProbeValu     ? reinterpret_cast<const void *>(&probe_valu)
-                    : probe == kProbePhilox ? reinterpret_cast<const void *>(&probe_philox)
-                                            : reinterpret_cast<const void *>(&probe_lds);
+    const void *kernels[kProbeCount] = {
+        reinterpret_cast<const void *>(&probe_valu),   reinterpret_cast<const void *>(&probe_philox),
+        reinterpret_cast<const void *>(&probe_lds),    reinterpret_cast<const void *>(&probe_mixed),
+        reinterpret_cast<const void *>(&probe_valu16), reinterpret_cast<const void *>(&probe_lds_row)};
+    const void *k = kernels[probe];
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, 0) != hipSuccess || per_cu <= 0)
         return fail(CR_ERR_DEVICE, "occupancy query failed for probe %u", probe);
@@ -115,8 +190,11 @@ extern "C" cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t 
     CR_HIP(hipStreamSynchronize(ctx->stream));
     float ms = 0;
     CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    const double per_lane_iter = probe == kProbeValu ? 2.0 * kValuChains : probe == kProbePhilox ? 4 * 60.0 : kLdsRows;
-    const double units = (double)waves * 64.0 * (double)iters * per_lane_iter;
+    // units per lane and loop iteration (VALU instructions of the loop body, or
+    // LDS increments)
+    const double per_lane_iter[kProbeCount] = {2.0 * kValuChains, 4 * 60.0, (double)kLdsRows,
+                                               2 * (60.0 + 4 * 3), 32.0, (double)kLdsRows};
+    const double units = (double)waves * 64.0 * (double)iters * per_lane_iter[probe];
     *units_per_s = ms > 0 ? units / (ms * 1e-3) : 0;
     if (kernel_ms) *kernel_ms = ms;
     return CR_OK;

@@ -656,7 +656,11 @@ Statistics cpu_stats(double total_ms) {
 
 // Reported-form Extended Hamming distribution of hamming_cpu<n> (hamming.cpp:43-93)
 // for n in {8, 16, 24, 32, 40}: the non-extended weights of all 2^n data
-// words, merged, with the 1-bit spheres, bins >= 3 shifted by n.  apply_w<N>
+// words, then the CPU driver's own post-processing, which differs from the
+// GPU driver's (hamming.cu:183-208) where the code has words of weight 1 or
+// 2: bins below 2 are not summed, bin 2 keeps the weight-2 count unshifted
+// (only bins >= 3 are shifted by n), odd weights merge into the next even bin
+// from bin 4 on, and the sphere of bin 3 uses bin 2.  apply_w<N>
 // (hamming.cpp:25-41) takes 4, 5 or 6 of the 32-bit parity masks; its masks
 // see only the low 32 data bits, so for n = 40 the weight of a word is that
 // of its low 32 bits under the (39,32) code plus the popcount of its high 8
@@ -677,10 +681,23 @@ void hamming_cpu_counts(cr_context *ctx, uint_t n, int with_1bit, uint128_t *cou
             for (uint_t j = 0; j <= 8; ++j) w40[a + j] += w[a] * binom<uint64_t>(8, j);
         w.swap(w40);
     }
-    const uint_t NN = n + cr_hamming_parity_bits(n) + 1;
-    std::vector<uint64_t> rep(2 * (NN + 1));
-    check(cr_hamming_report(n, w.data(), with_1bit, rep.data()), "run_hamming_cpu");
-    for (uint_t i = 0; i < count_counts && i <= NN; ++i) counts[i] = ((uint128_t)rep[2 * i + 1] << 64) | rep[2 * i];
+    const uint_t h = n == 8 ? 5 : (n < 32 ? 6 : 7);
+    const uint_t len = n + h;  // codeword bits incl. the overall parity bit
+    const uint_t cc = std::min<uint_t>(count_counts, len + 1);
+    std::vector<uint128_t> c(len + 1, 0);
+    for (uint_t d = 2; d < cc && d < w.size(); ++d) c[d] = w[d];
+    c[0] = (uint128_t)1 << n;
+    c[1] = with_1bit ? (uint128_t)len * c[0] : 0;
+    for (uint_t d = 4; d < cc; d += 2) {
+        c[d] += c[d - 1];
+        c[d - 1] = 0;
+    }
+    if (with_1bit)
+        for (uint_t d = 3; d < cc; d += 2)
+            c[d] = d < len ? (uint128_t)(len - d + 1) * c[d - 1] + (uint128_t)(d + 1) * c[d + 1]
+                           : (uint128_t)(len - d + 1) * c[d - 1];
+    for (uint_t d = 3; d < cc; ++d) c[d] <<= n;
+    for (uint_t d = 0; d < cc; ++d) counts[d] = c[d];
 }
 
 // The brute-force drivers (hamming_native.cpp, hamming_native_short.cpp,

@@ -69,12 +69,16 @@ cr_status cr_device_memory(cr_context *ctx, uint64_t *free_bytes, uint64_t *tota
  * counterpart in the reference, which times its kernels only).  Runs one
  * probe kernel on every resident single-wave workgroup for `iters` loop
  * iterations and returns the rate it sustained:
- *   probe 0: int32 VALU lane-ops/s of xor + popcount-accumulate (the inner
- *            loop of the pair, grid and exhaustive Hamming kernels);
- *   probe 1: Philox4x32-10 lane-ops/s, 60 per block (the Monte Carlo
- *            generator: 20 64-bit multiplies and 40 xors per block);
- *   probe 2: lane increments/s of wave-wide ds_add_u32 to lane-private LDS
- *            counters in the histogram layout of every kernel.
+ *   probe 0, 4: VALU lane-instructions/s of xor + popcount-accumulate on 8 or
+ *               16 independent chains (the pair, grid and exhaustive Hamming
+ *               inner loops);
+ *   probe 1:    VALU lane-instructions/s of Philox4x32-10 (the Monte Carlo
+ *               generator: 20 64-bit multiplies and 40 xors per block);
+ *   probe 3:    the same for Philox blocks whose words go through the AN
+ *               Monte Carlo pair work (multiply, xor, popcount);
+ *   probe 2, 5: lane increments/s of wave-wide ds_add_u32 to lane-private LDS
+ *               counters in the histogram layout of every kernel, the lanes
+ *               of an instruction in different rows (2) or one row (5).
  * kernel_ms (may be NULL) receives the probe's device time. */
 cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t iters, double *units_per_s, double *kernel_ms);
 

@@ -36,6 +36,34 @@ HAMMING_PATTERNS = [0xAB55555556AAAD5B, 0xCD9999999B33366D, 0x78F1E1E1E3C3C78E, 
                     0x01FFFE0003FFF800, 0x01FFFFFFFC000000, 0xFE00000000000000]
 
 
+def macwilliams_nonext_weights(k, masks):
+    """Weight distribution of the non-extended code (data bits plus the parity
+    bits parity(d & mask_t)) from the MacWilliams identity: its dual is
+    spanned by the rows [mask_t | e_t]."""
+    dm = (1 << k) - 1
+    masks = [mk & dm for mk in masks if mk & dm]
+    N = k + len(masks)
+    rows = [mk | (1 << (k + t)) for t, mk in enumerate(masks)]
+    B = [0] * (N + 1)
+    for sel in range(1 << len(rows)):
+        v = 0
+        for i, row in enumerate(rows):
+            if sel >> i & 1:
+                v ^= row
+        B[bin(v).count("1")] += 1
+
+    def kraw(w, j):
+        return sum((-1) ** s * math.comb(j, s) * math.comb(N - j, w - s) for s in range(0, w + 1))
+
+    out = []
+    for w in range(N + 1):
+        tot = sum(B[j] * kraw(w, j) for j in range(N + 1))
+        assert tot % (1 << len(rows)) == 0
+        out.append(tot >> len(rows))
+    assert sum(out) == 1 << k
+    return out
+
+
 def macwilliams_weights(k, masks=None):
     """Extended-codeword weight distribution A_w of the (N,k) code whose
     parity bits are parity(d & mask_t), from the MacWilliams identity of its

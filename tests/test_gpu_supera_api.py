@@ -77,12 +77,14 @@ def test_exhaustive_k8_sweep_against_oracle_and_recorded_search():
     fx = [f for f in golden("supera_runs.json")["files"] if f["src"].endswith("superA_n8_h6.txt")][0]
     assert [tuple(x) for x in fx["rows"]] == lines
     # process_result rows: header, then one row per A with the hist columns
+    # hist_1 .. hist_{k+h+1} (super_a/src/globals.cpp:75-89 reads one bin past
+    # the histogram, the zero of the caller's COUNTS_MAX_WIDTH array)
     assert csv[0].startswith("id, k, A, h, A_2, m_1, m_2, hlen")
     for line, A in zip(csv[1:], As):
         v = line.split(",")
         cnts = int(v[7])
         assert int(v[2]) == A and cnts == k + A.bit_length() + 1
-        assert [int(x) for x in v[8 + cnts: 8 + 2 * cnts]] == rows[A]["counts"][1:cnts + 1]
+        assert [int(x) for x in v[8 + cnts: 8 + 2 * cnts]] == rows[A]["counts"][1:] + [0]
 
 
 def test_exhaustive_k16_recorded_minimum():
@@ -129,10 +131,13 @@ def test_grid1d_against_oracle_and_recorded_file():
 
 
 def test_grid2d_against_oracle():
-    k, m1, m2 = 12, 100, 50
+    k, m1, m2 = 16, 100, 50
     rows, errors, _ = call("grid", 2, k, m1, m2, 61, 58659)
     assert not errors
     for A in (61, 58659):
         _, rep, _ = O.an_grid_ref(2, k, m1, m2, A)
         got = rows[A]["counts"]
         assert got == rep[: len(got)], A
+    # the driver requires A < 2^k (super_a/src/an_coding_grid.cu:182 asserts it)
+    rows, errors, _ = call("grid", 2, 12, m1, m2, 58659)
+    assert 58659 in errors and not rows

@@ -274,19 +274,56 @@ def _check_wide_counts(path, want):
             assert int(r) == w, i
 
 
+def hamming_cpu_reported(n, wnon, with_1bit):
+    """hamming_cpu<N>'s post-processing (hamming.cpp:59-92) of a non-extended
+    weight distribution: bins 0 and 1 set, bin 2 the weight-2 count as is,
+    odd weights merged into the next even bin from bin 4, the 1-bit spheres,
+    bins >= 3 shifted by n."""
+    h = 5 if n == 8 else (6 if n < 32 else 7)
+    cc = n + h + 1
+    c = [0] * cc
+    for d in range(2, cc):
+        c[d] = wnon[d] if d < len(wnon) else 0
+    c[0] = 1 << n
+    c[1] = (n + h) * c[0] if with_1bit else 0
+    for d in range(4, cc, 2):
+        c[d] += c[d - 1]
+        c[d - 1] = 0
+    if with_1bit:
+        for d in range(3, cc, 2):
+            c[d] = (n + h - d + 1) * c[d - 1] + ((d + 1) * c[d + 1] if d < n + h else 0)
+    return c[:3] + [x << n for x in c[3:]]
+
+
+def test_cpu_post_processing_equals_extended_distribution():
+    # for codes without words of weight 1 or 2 the CPU driver's post-processing
+    # of the non-extended weights is the reported form of the extended code's
+    # distribution (both MacWilliams, independent duals)
+    from conftest import HAMMING_PATTERNS, hamming_reported_from_ext, macwilliams_nonext_weights, macwilliams_weights
+    for n in (8, 16, 24, 32):
+        wnon = macwilliams_nonext_weights(n, HAMMING_PATTERNS)
+        for b in (0, 1):
+            assert hamming_cpu_reported(n, wnon, b) == hamming_reported_from_ext(n, macwilliams_weights(n), b)
+
+
 @pytest.mark.gpu
 @pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
 def test_cli_hamming_cpu_entry_point_n40(tmp_path):
     # run_hamming_cpu for n = 40: apply_w<40> (hamming.cpp:25-41) takes the
-    # six 32-bit parity masks, a (47,40) code whose distribution the
-    # MacWilliams identity of its dual gives exactly; counts above 2^64 are
-    # printed with 12 significant digits (globals.cpp:204-207)
-    from conftest import hamming_reported_from_ext, macwilliams_weights
+    # six 32-bit parity masks, so the 8 high data bits are unprotected and the
+    # code has words of weight 1 and 2, where the CPU driver's post-processing
+    # differs from the GPU driver's; the non-extended weights come from the
+    # MacWilliams identity of the code's dual; counts above 2^64 are printed
+    # with 12 significant digits (globals.cpp:204-207)
+    from conftest import macwilliams_nonext_weights
     n = 40
     masks32 = [0x56AAAD5B, 0x9B33366D, 0xE3C3C78E, 0x03FC07F0, 0x03FFF800, 0xFC000000]
-    want = hamming_reported_from_ext(n, macwilliams_weights(n, masks32), True)
-    run_cli(tmp_path, "-e", "1", "-c", "-n", str(n), "-b", "-f")
-    _check_wide_counts(tmp_path / f"hamming_cpu_n{n}_h7.csv", want)
+    wnon = macwilliams_nonext_weights(n, masks32)
+    assert wnon[1] == 8 and wnon[2] == 28
+    for b in (0, 1):
+        want = hamming_cpu_reported(n, wnon, b)
+        run_cli(tmp_path, "-e", "1", "-c", "-n", str(n), "-f", *(["-b"] if b else []))
+        _check_wide_counts(tmp_path / f"hamming_cpu_n{n}_h7.csv", want)
 
 
 @pytest.mark.gpu
@@ -298,7 +335,8 @@ def test_cli_an_mc_beyond_32_bits(tmp_path):
     run_cli(tmp_path, "-a", "61", "-n", "40", "-m", "2", "-f")
     rows, summary = _text(tmp_path / "ancoding_mc_gpu_m2_A61_n40_h6.csv")
     assert len(rows) == 47
-    assert summary[0] == '"Total",' + f"{float(4 ** 40):.6g}"
+    # the error columns leave the stream at 7 significant digits
+    assert summary[0] == '"Total",' + f"{float(4 ** 40):.7g}"
     counts = [float(r.split(",")[1]) for r in rows]
     assert counts[1] == 0  # no two multiples of 61 differ in one bit
     assert abs(sum(counts) / 4 ** 40 - 1) < 1e-9
