@@ -92,6 +92,7 @@ SIGNATURES = {
     "cr_an_pairs": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, u64p, C.c_uint32, C.POINTER(C.c_double)]),
     "cr_an_pairs_tuned": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, u64p, C.c_uint32,
                                     C.POINTER(cr_pair_tuning), C.POINTER(C.c_double)]),
+    "cr_an_pairs_noalloc": (C.c_int, [C.c_int, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p]),
     "cr_an_pairs_spectral": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, u64p, C.c_uint32,
                                        C.POINTER(C.c_double)]),
     "cr_an_grid_ref": (C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint64, u64p,

@@ -58,13 +58,7 @@ void HostBuffer::release() {
     bytes = 0;
 }
 
-// Single-code launches below this many pairs run with a static assignment.
-constexpr long double kStaticPairs = 1e9;
-
-// Tuning for one pair job when the caller leaves a knob at 0: enough items
-// (about 16 per resident wave) to balance the persistent waves, rows per lane
-// large enough to amortise the per-item set-up.
-static void auto_pair_tuning(uint64_t N, uint32_t resident_waves, uint32_t &R, uint32_t &Q) {
+void auto_pair_tuning(uint64_t N, uint32_t resident_waves, uint32_t &R, uint32_t &Q) {
     // rows per lane: small codes favour more, smaller row blocks (measured on
     // single-code launches: R = 2 up to 2^14 words, 4 at 2^15, 8 from 2^16)
     if (R == 0) R = N >= (1ull << 16) ? 8 : (N >= (1ull << 15) ? 4 : 2);
@@ -120,6 +114,9 @@ cr_status cr_context_create(int device, cr_context **out) {
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(CR_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", device, prop.gcnArchName);
     CR_HIP(hipSetDevice(device));
+    // the device's copy of the allocation-free entry point's scratch array; a
+    // failure here is reported by that entry point, which retries the look-up
+    if (noalloc_setup_device(device) != CR_OK) clear_error();
     cr_context *c = new cr_context();
     c->device = device;
     c->cus = prop.multiProcessorCount;

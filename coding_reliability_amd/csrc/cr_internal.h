@@ -91,6 +91,20 @@ __host__ __device__ inline uint64_t pair_prefix(uint64_t b, uint64_t NC, uint64_
     return b * NC - pair_floor_sum(b, Q);
 }
 
+// Single-code launches below this many pairs run with a static assignment
+// (about 100 us of work at 9.6e12 pairs/s: the queue's serialized atomics,
+// one per item, would cost more than a static assignment's imbalance).
+constexpr long double kStaticPairs = 1e9;
+
+// Tuning for one pair job when the caller leaves a knob at 0: enough items
+// (about 16 per resident wave) to balance the persistent waves, rows per lane
+// large enough to amortise the per-item set-up.
+void auto_pair_tuning(uint64_t N, uint32_t resident_waves, uint32_t &R, uint32_t &Q);
+
+// Looks up (once per device, with that device current) the address of the
+// allocation-free entry point's file-scope scratch array on `device`.
+cr_status noalloc_setup_device(int device);
+
 // Fills job geometry for R rows per lane and chunk width Q (in row blocks).
 void pair_job_geometry(PairJob &job, uint32_t R, uint32_t Q);
 // Bit length of the largest codeword (N-1)*A, i.e. the highest possible distance.

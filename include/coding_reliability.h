@@ -112,6 +112,23 @@ typedef struct cr_pair_tuning {
 cr_status cr_an_pairs_tuned(cr_context *ctx, uint64_t A, uint64_t num_words, uint64_t *raw, uint32_t nbins,
                             const cr_pair_tuning *tuning, double *kernel_ms);
 
+/* Allocation-free variant of cr_an_pairs_tuned (automatic tuning).  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc.  The pair kernel's job descriptor, work-queue head and
+ * histogram copies live in a file-scope device array of 32 KiB (one per GPU,
+ * held until the process ends, its address looked up once per device by
+ * cr_context_create or by the first call); the call allocates nothing.  It
+ * enqueues its kernels on `stream` (a hipStream_t; NULL = the null stream of
+ * `device`, which it makes current) and returns without waiting: raw_device
+ * (nbins uint64 in device memory) holds the raw histogram of cr_an_pairs once
+ * the stream reaches that point.  Every call on a GPU uses that GPU's array
+ * and starts by clearing it, so calls that overlap on one GPU race: the
+ * caller orders them (one stream, or events).  The histogram must fit the
+ * array: at most 65 bins (codewords of up to 64 bits). */
+cr_status cr_an_pairs_noalloc(int device, uint64_t A, uint64_t num_words, uint64_t *raw_device, uint32_t nbins,
+                              void *stream);
+
 /* Same raw[] as cr_an_pairs with num_words = 2^k, computed from the
  * Walsh-Hadamard spectrum of the code instead of its pairs: for the odd part
  * A' of A (the distances of A*i and A'*i agree) and n = bit length of
