@@ -176,7 +176,14 @@ bool spectral_preferred(uint64_t A, uint64_t num_words, uint32_t *k);
 struct SpectralPlan {
     uint64_t Ao = 0, tc = 0, chunks = 0, per_batch = 0, batches = 0, buffer_bytes = 0, slot_entries = 0;
     uint32_t k = 0, n = 0, nw = 0, Ainv = 0, nslots = 0;
+    uint32_t layout = 0;          // kSpectralInt32 / kSpectralInt16Pairs (see spectral.hip)
+    double sec_per_element = 0;   // measured device seconds per transformed element
 };
+// Pass structures of the spectral path: index bits 0-15 (or 0-7 for k <= 16)
+// then the rest through an int32 (int16) buffer; or, for k = 24 and at least
+// two transforms, bits 0-13 of two transforms at once then bits 14-23,
+// through int16.
+constexpr uint32_t kSpectralInt32 = 0, kSpectralInt16Pairs = 1;
 // False when (A, k) is outside the spectral path (then the pair kernel applies).
 bool spectral_plan(uint64_t A, uint32_t k, int cus, SpectralPlan *p);
 // Launches batches [first_batch, first_batch + nbatches) on stream, with X of

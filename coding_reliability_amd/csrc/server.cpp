@@ -197,8 +197,9 @@ cr_status cr_server::iterate(std::vector<Request *> &run) {
     long double pair_units = 0, ham_units = 0;
     for (Request *r : run) {
         if (r->spectral) {
-            // batches for the share at 2.2e-9 ms per transformed element
-            const long double batch_ms = (long double)r->sp.per_batch * (long double)(1ull << r->sp.k) * 2.2e-9L;
+            // batches for the share at the plan's measured time per transformed element
+            const long double batch_ms =
+                (long double)r->sp.per_batch * (long double)(1ull << r->sp.k) * (long double)r->sp.sec_per_element * 1e3L;
             uint64_t n = std::max<uint64_t>((uint64_t)(share / batch_ms), 1);
             n = std::min<uint64_t>(n, r->total_items - r->cursor);
             spec.push_back({r, n});

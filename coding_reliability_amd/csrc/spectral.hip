@@ -359,6 +359,184 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
     flush_slot(bins, slots + (uint64_t)blockIdx.x * 4 * nbins, nbins);
 }
 
+// ---------------------------------------------------------------------------
+// k = 24 through int16 (2 bytes of traffic per element each way): the first
+// pass transforms index bits 0-13 (values within +-2^14 fit int16) of two
+// chunks at once, the last pass the remaining 10 bits.
+//
+// spec_gen14p: workgroup (G, y) of 4 waves takes the 2^14 elements x_lo =
+// G * 2^14 + e of chunks sa = s0 + 2y (even) and sa + 1, each dword of its
+// working set holding the two chunks' values as int16 halves (packed
+// v_pk_add_u16 / v_pk_sub_i16 butterflies: one instruction per two values).
+// Since sa + 1 differs from sa in bit 0 only, parity((sa + 1) & h) =
+// parity(sa & h) ^ (h & 1).  Thread (wave w, lane l) generates e = w * 4096
+// + l * 64 + j (j < 64): bits 0-5 in registers; bits 6-11 after a transpose
+// within the wave through LDS (lane l, register j at l * 64 + (j ^ l):
+// conflict free both ways); bits 12-13 after an exchange across the waves in
+// two rounds of 32 registers.  Each chunk's result is written as int16 in the
+// layout Y[c][S_lo >> 6][x_hi][S_lo & 63] (x_hi < 2^(k-14)), so that a tile of
+// the last pass -- all x_hi of 32 consecutive S_lo -- is one contiguous run of
+// 64-byte rows.
+typedef short v2s __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2s, a) + __builtin_bit_cast(v2s, b));
+}
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2s, a) - __builtin_bit_cast(v2s, b));
+}
+// Packed butterflies over the register-index bits of strides [H0, H1).
+template <int H0, int H1>
+__device__ __forceinline__ void wht_pk(uint32_t (&v)[64]) {
+#pragma unroll
+    for (int h = H0; h < H1; h <<= 1)
+#pragma unroll
+        for (int j = 0; j < 64; ++j)
+            if ((j & h) == 0) {
+                const uint32_t a = v[j], b = v[j + h];
+                v[j] = pk_add(a, b);
+                v[j + h] = pk_sub(a, b);
+            }
+}
+
+constexpr uint32_t kGen14Threads = 256;
+__global__ __launch_bounds__(kGen14Threads, 2) void spec_gen14p(int16_t *__restrict__ Y, uint32_t k, uint32_t A,
+                                                                 uint32_t Ainv, uint32_t tc, uint32_t s0) {
+    __shared__ uint32_t L[4 * 64 * 64];  // 64 KB: a 64 x 64 transpose per wave, then the exchange
+    const uint32_t t = threadIdx.x, l = t & 63, w = t >> 6;
+    const uint32_t mask = (1u << k) - 1;
+    const uint32_t G = blockIdx.x;
+    const uint32_t sa = s0 + 2 * blockIdx.y;
+    const uint32_t x0 = (G << 14) | (w << 12) | (l << 6);
+    const uint32_t i = (uint32_t)((uint64_t)x0 * Ainv) & mask;
+    uint32_t h = (uint32_t)(((uint64_t)A * i) >> k);
+    uint32_t v[64];
+#pragma unroll
+    for (int j = 0; j < 64; ++j) {
+        const uint32_t pa = __popc(h & sa);
+        const uint32_t q = (pa | ((pa ^ h) << 16)) & 0x00010001u;  // parities of chunks sa (low) and sa+1 (high)
+        v[j] = 0x00010001u ^ (q * 0xFFFEu);                       // per half: 0 -> +1 (0x0001), 1 -> -1 (0xFFFF)
+        h = next_high(h, tc, A);
+    }
+    wht_pk<1, 64>(v);  // bits 0-5; |v| <= 2^6
+    uint32_t *R = L + w * 4096;
+#pragma unroll
+    for (int j = 0; j < 64; ++j) R[l * 64 + ((uint32_t)j ^ l)] = v[j];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 64; ++r) v[r] = R[(uint32_t)r * 64 + (l ^ (uint32_t)r)];
+    wht_pk<1, 64>(v);  // bits 6-11 (v[r]: bits 6-11 = r, bits 0-5 = l); |v| <= 2^12
+    // bits 12-13: after round hr, v[32 hr + 8 q + rr] holds register
+    // 32 hr + 8 w + rr of wave q
+#pragma unroll
+    for (int hr = 0; hr < 2; ++hr) {
+        __syncthreads();
+#pragma unroll
+        for (int rr = 0; rr < 32; ++rr) L[(w * 32 + rr) * 64 + l] = v[32 * hr + rr];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int rr = 0; rr < 8; ++rr) v[32 * hr + 8 * q + rr] = L[(q * 32 + 8 * w + rr) * 64 + l];
+    }
+    wht_pk<8, 32>(v);  // over q; |v| <= 2^14
+    // stores through one buffer resource per chunk (2^k int16): the lane part
+    // of the offset in a VGPR, the rest wave-uniform
+    const uint32_t X = 1u << (k - 14);
+    int16_t *Ya = Y + ((uint64_t)(2 * blockIdx.y) << k);
+    const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(Ya, 0, (int)(2u << k), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rb =
+        __builtin_amdgcn_make_buffer_rsrc(Ya + ((uint64_t)1 << k), 0, (int)(2u << k), 0x00020000);
+    const uint32_t wu = __builtin_amdgcn_readfirstlane(w);
+    const int voff = (int)((G * 64 + l) * 2);
+#pragma unroll
+    for (int hr = 0; hr < 2; ++hr)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int rr = 0; rr < 8; ++rr) {
+                const uint32_t sb = ((uint32_t)q << 6) | (32u * hr + 8u * wu + rr);  // S_lo >> 6
+                const int soff = (int)(sb * X * 128u);                              // bytes of sb * X rows of 64
+                const uint32_t x = v[32 * hr + 8 * q + rr];
+                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(x & 0xFFFFu), ra, voff, soff, 0);
+                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(x >> 16), rb, voff, soff, 0);
+            }
+}
+
+// spec_last10: tile (c, sb, half) = rows x_hi < 1024 and the 32 columns S_lo =
+// sb * 64 + half * 32 + cc of chunk c (64 KB of int16).  Thread (row group g <
+// 16, column cc) loads rows g * 64 + j (buffer loads: tile base in the
+// resource, row offset a scalar), transforms x_hi bits 0-5 in registers, then
+// bits 6-9 after four exchange rounds through LDS (round p: every thread
+// stores registers 16 p .. 16 p + 15 and gathers register 16 p + g of all 16
+// groups; the two groups of a wave store to opposite 32-dword halves of a row,
+// alternating with the register index, so stores and gathers are conflict
+// free), and sums F^2 by weight |s| + |S_lo| + |S_hi| as spec_last does.
+// Tiles are visited so that the two column halves of a row segment go to
+// workgroups b and b + 8, which run on the same XCD (workgroups are dealt to
+// the 8 XCDs in turn) at the same time.
+constexpr uint32_t kLast10Threads = 512;
+__global__ __launch_bounds__(kLast10Threads) void spec_last10(const int16_t *__restrict__ Y, uint32_t k, uint64_t s0,
+                                                              uint32_t nchunks, unsigned long long *__restrict__ slots,
+                                                              uint32_t nbins) {
+    __shared__ uint32_t E[8 * 16 * 64];  // 32 KB
+    __shared__ unsigned long long bins[2 * 64];
+    const uint32_t t = threadIdx.x, cc = t & 31, g = t >> 5;
+    for (uint32_t b = t; b < 2 * 64; b += kLast10Threads) bins[b] = 0;
+    __syncthreads();
+    const uint32_t X = 1u << (k - 14);  // rows x_hi: 1024 at k = 24
+    const uint64_t ntiles = (uint64_t)nchunks * 512;
+    for (uint64_t u = blockIdx.x; u < ntiles; u += gridDim.x) {
+        const uint64_t jx = u >> 3;
+        const uint32_t half = (uint32_t)(jx & 1);
+        const uint64_t P = ((jx >> 1) << 3) | (u & 7);
+        const uint32_t c = (uint32_t)(P >> 8), sb = (uint32_t)(P & 255);
+        const int16_t *tile = Y + ((uint64_t)c << k) + (uint64_t)sb * X * 64 + half * 32;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<int16_t *>(tile), 0, (int)(((X - 1) * 64 + 32) * 2), 0x00020000);
+        int32_t v[64];
+#pragma unroll
+        for (int j = 0; j < 64; ++j)
+            v[j] = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(rs, (int)((g * 64 * 64 + cc) * 2), j * 128, 0);
+        wht64(v);  // x_hi bits 0-5; |v| <= 2^20
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            __syncthreads();
+#pragma unroll
+            for (int jj = 0; jj < 16; ++jj)
+                E[((g >> 1) * 16 + jj) * 64 + ((((g & 1) ^ (uint32_t)jj) & 1) << 5) + cc] = (uint32_t)v[16 * p + jj];
+            __syncthreads();
+#pragma unroll
+            for (int gg = 0; gg < 16; ++gg)
+                v[16 * p + gg] = (int32_t)E[(((uint32_t)gg >> 1) * 16 + g) * 64 + ((((uint32_t)gg ^ g) & 1) << 5) + cc];
+        }
+        // v[16 p + gg]: x_hi bits 6-9 = gg, transformed bits 0-5 = 16 p + g
+#pragma unroll
+        for (int hb = 1; hb < 16; hb <<= 1)
+#pragma unroll
+            for (int q = 0; q < 64; ++q)
+                if ((q & hb) == 0) {
+                    const int32_t a = v[q], b = v[q + hb];
+                    v[q] = a + b;
+                    v[q + hb] = a - b;
+                }
+        // |F| <= 2^24: a thread's 64 squares stay below 2^54 in one register pair
+        unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 64; ++q) {
+            const int64_t f = v[q];
+            acc[__builtin_popcount(q >> 4) + __builtin_popcount(q & 15)] += (unsigned long long)(f * f);
+        }
+        const uint32_t base = (uint32_t)__popcll(s0 + c) + __popc(sb) + half + __popc(cc) + __popc(g);
+#pragma unroll
+        for (int o_ = 0; o_ < 7; ++o_) {
+            atomicAdd(&bins[2 * (base + o_)], acc[o_] & 0xFFFFFFFFull);
+            atomicAdd(&bins[2 * (base + o_) + 1], acc[o_] >> 32);
+        }
+    }
+    __syncthreads();
+    flush_slot(bins, slots + (uint64_t)blockIdx.x * 4 * nbins, nbins);
+}
+
 // Narrow last pass: the R < 8 top index bits [tbit, tbit + R) of every chunk,
 // tiles of 2^R rows (strided by 2^tbit) x 256 contiguous columns, thread t
 // owning column t and all 2^R rows in registers (no exchange).  Element
@@ -438,13 +616,27 @@ cr_status launch_narrow(uint32_t R, const In *X, uint32_t k, uint32_t tbit, uint
 
 size_t spectral_tile_count(uint32_t k) { return (size_t)1 << (k - 14); }
 
-cr_status launch_spectral_batch(void *X, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc, uint64_t s0,
-                                uint32_t nchunks, unsigned long long *slots, uint32_t nslots, uint32_t nbins,
-                                hipStream_t stream) {
+cr_status launch_spectral_batch(void *X, uint32_t layout, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc,
+                                uint64_t s0, uint32_t nchunks, unsigned long long *slots, uint32_t nslots,
+                                uint32_t nbins, hipStream_t stream) {
     if (k < 14 || k > 32) return fail(CR_ERR_UNSUPPORTED, "spectral path: k=%u outside [14, 32]", k);
     if (nbins > 64) return fail(CR_ERR_UNSUPPORTED, "spectral path: %u weights", nbins);
     if ((A >> 32) || (tc >> 32) || (s0 + nchunks) > (1ull << 32))
         return fail(CR_ERR_UNSUPPORTED, "spectral path: multiplier or transform count beyond 32 bits");
+    if (layout == kSpectralInt16Pairs) {
+        // X: nchunks * 2^k int16 after index bits 0-13, in pairs of transforms
+        if (k != 24 || (nchunks & 1) || (s0 & 1) || (nslots & 15))
+            return fail(CR_ERR_INVALID_ARGUMENT, "spectral int16 pairs: k=%u, %u transforms from %llu, %u slots", k,
+                        nchunks, (unsigned long long)s0, nslots);
+        int16_t *y = static_cast<int16_t *>(X);
+        hipLaunchKernelGGL(spec_gen14p, dim3(1u << (k - 14), nchunks / 2), dim3(kGen14Threads), 0, stream, y, k,
+                           (uint32_t)A, Ainv, (uint32_t)tc, (uint32_t)s0);
+        CR_HIP(hipGetLastError());
+        hipLaunchKernelGGL(spec_last10, dim3(nslots), dim3(kLast10Threads), 0, stream, (const int16_t *)y, k, s0,
+                           nchunks, slots, nbins);
+        CR_HIP(hipGetLastError());
+        return CR_OK;
+    }
     const uint32_t tiles = (uint32_t)spectral_tile_count(k);
     if (k <= 16) {
         // X: nchunks * 2^k int16 after index bits 0-7 (tiles of 64 transforms: k >= 14)

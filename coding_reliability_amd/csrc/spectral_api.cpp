@@ -12,9 +12,9 @@
 namespace cr {
 
 size_t spectral_tile_count(uint32_t k);
-cr_status launch_spectral_batch(void *X, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc, uint64_t s0,
-                                uint32_t nchunks, unsigned long long *slots, uint32_t nslots, uint32_t nbins,
-                                hipStream_t stream);
+cr_status launch_spectral_batch(void *X, uint32_t layout, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc,
+                                uint64_t s0, uint32_t nchunks, unsigned long long *slots, uint32_t nslots,
+                                uint32_t nbins, hipStream_t stream);
 
 namespace {
 
@@ -72,10 +72,9 @@ bool spectral_preferred(uint64_t A, uint64_t num_words, uint32_t *k) {
     uint64_t Ao = 0;
     uint32_t n = 0;
     if (!spectral_shape(A, kk, &Ao, &n)) return false;
-    // measured seconds per transformed element: two passes up to k = 24
-    // (2^40 elements in 2.35 s), three beyond (2^38 in 1.03 s at k = 32)
-    const double per_element = kk <= 24 ? 2.2e-12 : 3.8e-12;
-    const double spectral_s = std::ldexp(per_element, (int)n) + 50e-6;
+    SpectralPlan p;
+    if (!spectral_plan(A, kk, 256, &p)) return false;
+    const double spectral_s = std::ldexp(p.sec_per_element, (int)n) + 50e-6;
     const double pair_s = std::ldexp(1.0, 2 * (int)kk - 1) / 9.6e12;
     *k = kk;
     return spectral_s < 0.8 * pair_s;
@@ -93,10 +92,17 @@ bool spectral_plan(uint64_t A, uint32_t k, int cus, SpectralPlan *p) {
     p->Ainv = inverse_mod_2_32((uint32_t)Ao) & (uint32_t)(N - 1);
     p->tc = (Ao * p->Ainv - 1) >> k;  // Ao * Ainv = 1 + tc * 2^k (below 2^56)
     p->chunks = 1ull << (n - k);
+    // k = 24 with at least two transforms: pairs of transforms through int16
+    // (an even number per batch, starting at an even index)
+    p->layout = (k == 24 && p->chunks >= 2) ? kSpectralInt16Pairs : kSpectralInt32;
+    // measured seconds per transformed element: two passes up to k = 24
+    // (2^40 elements in 2.35 s through int32), three beyond (2^38 in 1.03 s at k = 32)
+    p->sec_per_element = p->layout == kSpectralInt16Pairs ? 1.2e-12 : (k <= 24 ? 2.2e-12 : 3.8e-12);
     // chunks of 2^k per launch: 256 MB of transforms in flight, about the
-    // size of the last-level cache (measured at k = 24, A = 58659: 2.36 s with
-    // 256 MB, 2.63 s with 128 MB, 2.67 s with 512 MB)
-    const uint64_t chunk_bytes = N * (k <= 16 ? sizeof(int16_t) : sizeof(int32_t));
+    // size of the last-level cache (measured at k = 24, A = 58659 through
+    // int32: 2.36 s with 256 MB, 2.63 s with 128 MB, 2.67 s with 512 MB)
+    const uint64_t chunk_bytes =
+        N * ((k <= 16 || p->layout == kSpectralInt16Pairs) ? sizeof(int16_t) : sizeof(int32_t));
     p->per_batch = std::min<uint64_t>(p->chunks, std::max<uint64_t>(1, (1ull << 28) / chunk_bytes));
     p->batches = (p->chunks + p->per_batch - 1) / p->per_batch;
     p->buffer_bytes = p->per_batch * chunk_bytes;
@@ -110,7 +116,7 @@ cr_status spectral_run(const SpectralPlan &p, void *X, unsigned long long *slots
     for (uint64_t b = first_batch; b < first_batch + nbatches && b < p.batches; ++b) {
         const uint64_t s0 = b * p.per_batch;
         const uint32_t nb = (uint32_t)std::min<uint64_t>(p.per_batch, p.chunks - s0);
-        CR_TRY(launch_spectral_batch(X, p.k, p.Ao, p.Ainv, p.tc, s0, nb, slots, p.nslots, p.nw, stream));
+        CR_TRY(launch_spectral_batch(X, p.layout, p.k, p.Ao, p.Ainv, p.tc, s0, nb, slots, p.nslots, p.nw, stream));
     }
     return CR_OK;
 }
