@@ -213,82 +213,29 @@ def _kernel_entry(name: str, units: float, ms: float, unit: str, ceil: dict, wor
 
 def _ceilings_json(ceil: dict) -> dict:
     return {"valu": round(ceil["valu"] / 1e12, 4), "lds": round(ceil["lds"] / 1e12, 4),
-            "unit": "T/s: VALU lane-instructions, LDS lane increments (cr_probe_ceiling, this run)",
+            "write": round(ceil["write"] / 1e12, 4), "read": round(ceil["read"] / 1e12, 4),
+            "unit": "T/s: VALU lane-instructions, LDS lane increments, TB/s streaming stores and loads over 256 MiB "
+                    "(cr_probe_ceiling, this run)",
             "probes": {kk: round(v / 1e12, 4) for kk, v in ceil["probes"].items()}}
 
 
 def measure_ceilings(ctx) -> dict:
     """The device ceilings of this run (cr_probe_ceiling), each the best of
     three launches after one untimed one: VALU issue (the best of the
-    xor+popcount, Philox and mixed probes, in lane-instructions/s) and LDS
-    increments (lane increments/s, the better of the two lane patterns)."""
-    iters = {0: 1 << 15, 1: 1 << 12, 2: 1 << 14, 3: 1 << 12, 4: 1 << 14, 5: 1 << 14}
+    xor+popcount, Philox and mixed probes, in lane-instructions/s), LDS
+    increments (lane increments/s, the better of the two lane patterns) and
+    streaming stores and loads over a 256 MiB buffer (bytes/s)."""
+    iters = {0: 1 << 15, 1: 1 << 12, 2: 1 << 14, 3: 1 << 12, 4: 1 << 14, 5: 1 << 14, 6: 16, 7: 16}
 
     def best(probe):
         ctx.probe_ceiling(probe, iters[probe])
         return max(ctx.probe_ceiling(probe, iters[probe])[0] for _ in range(3))
 
-    r = {p: best(p) for p in range(6)}
-    return {"valu": max(r[0], r[1], r[3], r[4]), "lds": max(r[2], r[5]),
+    r = {p: best(p) for p in range(8)}
+    return {"valu": max(r[0], r[1], r[3], r[4]), "lds": max(r[2], r[5]), "write": r[6], "read": r[7],
             "probes": {"valu_xor_popcount_8": r[0], "valu_philox": r[1], "lds_scattered_rows": r[2],
-                       "valu_philox_pair_mix": r[3], "valu_xor_popcount_16": r[4], "lds_same_row": r[5]}}
-
-
-def _warm(f):
-    """Runs f twice (the first call loads the code object and warms the
-    caches) and returns the second call's result."""
-    f()
-    return f()
-
-
-def run_kernels(ctx, ceil: dict) -> dict:
-    """The other hot kernels on their workloads, each checked for a complete
-    histogram, timed with HIP events on the library's stream after one
-    untimed call."""
-    import coding_reliability_amd as cra
-    out = {}
-    k = 32
-    h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_RANGE, 0, 1 << k))
-    assert sum(h) == 1 << k
-    out["hamming_exhaustive_39_32"] = _kernel_entry("hamming_exhaustive_39_32", 1 << k, ms, "Gwords/s", ceil,
-                                                    "(39,32) weights of all 2^32 data words")
-    h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 20, stride=65537))
-    assert sum(h) == 1 << 20
-    e = _kernel_entry("hamming_grid_39_32", 1 << 20, ms, "Gwords/s", ceil,
-                      "(39,32) integer grid d = i*65537 mod 2^32, i < 2^20")
-    # 2^20 words are well under a microsecond of device work: the launch is
-    # latency-bound, so its floor (the same kernel over 4096 words) is given
-    _, floor_ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 12, stride=65537))
-    e["launch_floor_ms"] = round(floor_ms, 4)
-    out["hamming_grid_39_32"] = e
-    S = 1 << 34
-    h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_MC, 0, S, seed=2026, stream=0))
-    assert sum(h) == S
-    out["hamming_mc_39_32"] = _kernel_entry("hamming_mc_39_32", S, ms, "Gwords/s", ceil, "(39,32) 2^34 Philox words")
-    S = 1 << 30
-    h, ms = _warm(lambda: ctx.hamming_weights(64, cra.HAM_MC, 0, S, seed=2026, stream=1))
-    assert sum(h) == S
-    out["hamming_mc_72_64"] = _kernel_entry("hamming_mc_72_64", S, ms, "Gwords/s", ceil, "(72,64) 2^30 Philox words")
-    h, ms = _warm(lambda: ctx.an_grid_ref(1, 32, 1001, 0, 61))
-    pairs = sum(h)
-    out["an_grid1d_ref"] = _kernel_entry("an_grid1d_ref", pairs, ms, "Gpairs/s", ceil,
-                                         f"reference grid 1D, A=61, k=32, m=1001 (38-bit codewords, {pairs} pairs)")
-    m = 1 << 16
-    h, ms = _warm(lambda: ctx.an_mc(61, 24, m, 16384))
-    assert sum(h) == (1 << 24) * m
-    out["an_mc"] = _kernel_entry("an_mc", (1 << 24) * m, ms, "Gpairs/s", ceil,
-                                 "AN Monte Carlo, A=61, k=24, 2^16 samples")
-    best, cands, ms = _warm(lambda: ctx.an_candidates(24, 16))
-    # pruning skips most (A, d) steps, so the work is counted as the steps the
-    # candidates (A attaining the maximum, never pruned) must run in full: a
-    # lower bound of the executed steps, hence a lower bound of the fraction
-    steps = len(cands) * ((1 << 24) - 1)
-    e = _kernel_entry("an_candidates", steps, ms, "Gsteps/s", ceil,
-                      "ANcandidates cell k=24, |A|=16 (16384 odd A); rate = steps of the A attaining the maximum "
-                      "(each runs all 2^24-1 steps) per second, a lower bound")
-    e["best"], e["candidates"] = best, len(cands)
-    out["an_candidates"] = e
-    return out
+                       "valu_philox_pair_mix": r[3], "valu_xor_popcount_16": r[4], "lds_same_row": r[5],
+                       "stream_write_bytes": r[6], "stream_read_bytes": r[7]}}
 
 
 # Timings the reference records in its own result files (distance_distribution/
@@ -303,7 +250,7 @@ REFERENCE_RECORDED = {
 }
 
 
-def run_spectral(ctx, A: int, k: int, raw_pairs: list, pair_kernel_s: float) -> dict:
+def run_spectral(ctx, A: int, k: int, raw_pairs: list, pair_kernel_s: float, ceil: dict | None = None) -> dict:
     """The headline histogram again, from the code's Walsh-Hadamard spectrum
     (cr_an_pairs_spectral, the path cr_an_pairs takes for this code): one
     untimed call, one timed call, checked equal to the pair kernel's."""
@@ -316,17 +263,22 @@ def run_spectral(ctx, A: int, k: int, raw_pairs: list, pair_kernel_s: float) -> 
     Ao = A // (A & -A)  # odd part: the same distances
     n = max((Ao * (N - 1)).bit_length(), k)
     # memory roofline of the passes: each transformed element is written once
-    # as int32 by the first pass and read once by the last (8 bytes; k = 24)
+    # by the first pass and read once by the last: int16 for k = 24 (two
+    # transforms per first-pass workgroup), int32 for 17 <= k <= 23
     elements = 1 << n
-    bytes_per_element = 8 if 17 <= k <= 24 else None
+    bytes_per_element = 4 if (k == 24 and n > k) else (8 if 17 <= k <= 24 else None)
     memory = None
     if bytes_per_element:
         achieved = elements * bytes_per_element / (ms / 1e3) / 1e12
+        half = elements * bytes_per_element / 2
+        floor_s = half / ceil["write"] + half / ceil["read"] if ceil else None
         memory = {"bytes_per_element": bytes_per_element, "achieved": round(achieved, 3), "peak": HBM_PEAK_TBS,
-                  "unit": "TB/s", "frac": round(achieved / HBM_PEAK_TBS, 4),
-                  "note": "the first pass (the +-1 generation and 16 butterfly stages) issues VALU at about half "
-                          "the one-per-clock rate, one 1024-thread workgroup per CU idling at its barriers; the "
-                          "last is bound by memory"}
+                  "unit": "TB/s", "frac": round(achieved / HBM_PEAK_TBS, 4)}
+        if floor_s:
+            memory.update({"measured_stream_floor_s": round(floor_s, 4),
+                           "frac_of_measured_stream": round(floor_s / (ms / 1e3), 4),
+                           "note": "floor = the bytes written at the streaming-store ceiling plus the bytes read at "
+                                   "the streaming-load ceiling, both measured in this run over 256 MiB"})
     return {
         "method": "Walsh-Hadamard spectrum of the code: 2^(n-k) transforms of length 2^k, exact integer sums, "
                   "Krawtchouk transform on the host",
@@ -665,7 +617,7 @@ def main() -> int:
     spectral = None
     if not args.no_spectral:
         log("spectral ...")
-        spectral = run_spectral(ctx, A, k, raw, avg_kernel_s)
+        spectral = run_spectral(ctx, A, k, raw, avg_kernel_s, ceil)
         log(f"spectral: {spectral['seconds']} s, {spectral['equivalent_gpairs_per_s']} Gpairs/s equivalent")
 
     batched = None

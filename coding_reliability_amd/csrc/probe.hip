@@ -16,6 +16,8 @@
 //                   pair work (multiply, xor, popcount): 72 per block
 //   4 kProbeValu16  as 0 on 16 chains
 //   5 kProbeLdsRow  as 2, all lanes of an instruction in one row
+//   6 kProbeWrite   streaming 16-byte stores over a 256 MiB buffer (bytes/s)
+//   7 kProbeRead    streaming 16-byte loads over the same buffer (bytes/s)
 #include "cr_device.h"
 #include "cr_internal.h"
 
@@ -25,7 +27,8 @@ namespace {
 using namespace dev;
 
 constexpr uint32_t kProbeValu = 0, kProbePhilox = 1, kProbeLds = 2, kProbeMixed = 3, kProbeValu16 = 4,
-                   kProbeLdsRow = 5, kProbeCount = 6;
+                   kProbeLdsRow = 5, kProbeWrite = 6, kProbeRead = 7, kProbeCount = 8;
+constexpr size_t kStreamBytes = 256ull << 20;
 constexpr int kValuChains = 8;
 constexpr int kLdsRows = 16;
 
@@ -159,6 +162,25 @@ __global__ __launch_bounds__(64) void probe_lds_row(uint32_t iters, uint32_t see
     out[blockIdx.x * 64 + lane] = t;
 }
 
+// Streaming stores / loads: `iters` passes over n uint4 with a grid stride.
+__global__ __launch_bounds__(256) void probe_write(uint32_t iters, uint32_t seed, uint4 *buf, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint4 x = make_uint4(seed, threadIdx.x, blockIdx.x, 0);
+    for (uint32_t it = 0; it < iters; ++it)
+        for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) buf[e] = x;
+}
+__global__ __launch_bounds__(256) void probe_read(uint32_t iters, uint32_t seed, const uint4 *buf, uint64_t n,
+                                                  uint32_t *out) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t acc = seed;
+    for (uint32_t it = 0; it < iters; ++it)
+        for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+            const uint4 v = buf[e];
+            acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        }
+    if (acc == 0x9E3779B9u) out[0] = acc;  // keeps the loads
+}
+
 }  // namespace
 }  // namespace cr
 
@@ -171,7 +193,28 @@ extern "C" cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t 
     if (probe >= kProbeCount) return fail(CR_ERR_INVALID_ARGUMENT, "unknown probe %u", probe);
     if (iters == 0 || iters > (1u << 24)) return fail(CR_ERR_INVALID_ARGUMENT, "iters=%u outside [1, 2^24]", iters);
     CR_HIP(hipSetDevice(ctx->device));
-    const void *kernels[kProbeCount] = {
+    if (probe == kProbeWrite || probe == kProbeRead) {
+        if (iters > 64) return fail(CR_ERR_INVALID_ARGUMENT, "streaming probes: iters=%u above 64 passes", iters);
+        CR_TRY(ctx->table_b.reserve(kStreamBytes + 256));
+        uint4 *buf = ctx->table_b.as<uint4>();
+        const uint64_t n = kStreamBytes / sizeof(uint4);
+        const uint32_t blocks = (uint32_t)ctx->cus * 8;
+        CR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        if (probe == kProbeWrite)
+            hipLaunchKernelGGL(probe_write, dim3(blocks), dim3(256), 0, ctx->stream, iters, 0x2545F491u, buf, n);
+        else
+            hipLaunchKernelGGL(probe_read, dim3(blocks), dim3(256), 0, ctx->stream, iters, 0x2545F491u,
+                               (const uint4 *)buf, n, reinterpret_cast<uint32_t *>(buf + n));
+        CR_HIP(hipGetLastError());
+        CR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        CR_HIP(hipStreamSynchronize(ctx->stream));
+        float ms = 0;
+        CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *units_per_s = ms > 0 ? (double)kStreamBytes * iters / (ms * 1e-3) : 0;
+        if (kernel_ms) *kernel_ms = ms;
+        return CR_OK;
+    }
+    const void *kernels[kProbeCount - 2] = {
         reinterpret_cast<const void *>(&probe_valu),   reinterpret_cast<const void *>(&probe_philox),
         reinterpret_cast<const void *>(&probe_lds),    reinterpret_cast<const void *>(&probe_mixed),
         reinterpret_cast<const void *>(&probe_valu16), reinterpret_cast<const void *>(&probe_lds_row)};
@@ -192,8 +235,8 @@ extern "C" cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t 
     CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     // units per lane and loop iteration (VALU instructions of the loop body, or
     // LDS increments)
-    const double per_lane_iter[kProbeCount] = {2.0 * kValuChains, 4 * 60.0, (double)kLdsRows,
-                                               2 * (60.0 + 4 * 3), 32.0, (double)kLdsRows};
+    const double per_lane_iter[kProbeCount - 2] = {2.0 * kValuChains, 4 * 60.0, (double)kLdsRows,
+                                                   2 * (60.0 + 4 * 3), 32.0, (double)kLdsRows};
     const double units = (double)waves * 64.0 * (double)iters * per_lane_iter[probe];
     *units_per_s = ms > 0 ? units / (ms * 1e-3) : 0;
     if (kernel_ms) *kernel_ms = ms;

@@ -479,9 +479,11 @@ __global__ __launch_bounds__(kLast10Threads) void spec_last10(const int16_t *__r
                                                               uint32_t nchunks, unsigned long long *__restrict__ slots,
                                                               uint32_t nbins) {
     __shared__ uint32_t E[8 * 16 * 64];  // 32 KB
-    __shared__ unsigned long long bins[2 * 64];
+    // weight bins per column lane: bins[(w * 2 + half) * 32 + cc] (lo, hi
+    // halves of F^2): the lanes of an add never share an address
+    __shared__ unsigned long long bins[64 * 2 * 32];  // 32 KB
     const uint32_t t = threadIdx.x, cc = t & 31, g = t >> 5;
-    for (uint32_t b = t; b < 2 * 64; b += kLast10Threads) bins[b] = 0;
+    for (uint32_t b = t; b < 64 * 2 * 32; b += kLast10Threads) bins[b] = 0;
     __syncthreads();
     const uint32_t X = 1u << (k - 14);  // rows x_hi: 1024 at k = 24
     const uint64_t ntiles = (uint64_t)nchunks * 512;
@@ -529,12 +531,24 @@ __global__ __launch_bounds__(kLast10Threads) void spec_last10(const int16_t *__r
         const uint32_t base = (uint32_t)__popcll(s0 + c) + __popc(sb) + half + __popc(cc) + __popc(g);
 #pragma unroll
         for (int o_ = 0; o_ < 7; ++o_) {
-            atomicAdd(&bins[2 * (base + o_)], acc[o_] & 0xFFFFFFFFull);
-            atomicAdd(&bins[2 * (base + o_) + 1], acc[o_] >> 32);
+            atomicAdd(&bins[((base + o_) * 2) * 32 + cc], acc[o_] & 0xFFFFFFFFull);
+            atomicAdd(&bins[((base + o_) * 2 + 1) * 32 + cc], acc[o_] >> 32);
         }
     }
     __syncthreads();
-    flush_slot(bins, slots + (uint64_t)blockIdx.x * 4 * nbins, nbins);
+    // the 32 column lanes' bins into the first column (one thread per (w, half))
+    for (uint32_t b = t; b < 2 * nbins; b += kLast10Threads) {
+        unsigned long long x = 0;
+        for (uint32_t q = 0; q < 32; ++q) x += bins[b * 32 + ((q + b) & 31)];
+        bins[b * 32] = x;
+    }
+    __syncthreads();
+    unsigned long long *slot = slots + (uint64_t)blockIdx.x * 4 * nbins;
+    for (uint32_t b = t; b < 2 * nbins; b += kLast10Threads) {
+        const unsigned long long x = bins[b * 32], old = slot[2 * b];
+        slot[2 * b] = old + x;
+        if (old + x < old) slot[2 * b + 1] += 1;
+    }
 }
 
 // Narrow last pass: the R < 8 top index bits [tbit, tbit + R) of every chunk,

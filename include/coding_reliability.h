@@ -78,7 +78,10 @@ cr_status cr_device_memory(cr_context *ctx, uint64_t *free_bytes, uint64_t *tota
  *               Monte Carlo pair work (multiply, xor, popcount);
  *   probe 2, 5: lane increments/s of wave-wide ds_add_u32 to lane-private LDS
  *               counters in the histogram layout of every kernel, the lanes
- *               of an instruction in different rows (2) or one row (5).
+ *               of an instruction in different rows (2) or one row (5);
+ *   probe 6, 7: bytes/s of streaming 16-byte stores (6) or loads (7), `iters`
+ *               passes (at most 64) over a 256 MiB buffer, the size of the
+ *               spectral path's batches.
  * kernel_ms (may be NULL) receives the probe's device time. */
 cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t iters, double *units_per_s, double *kernel_ms);
 

@@ -48,8 +48,19 @@ def test_pair_kernel_within_ceilings(ctx, ceilings):
     assert 3 * rate <= 1.03 * valu
 
 
+def test_streaming_probes(ctx):
+    # 16 passes of 16-byte stores / loads over 256 MiB, bytes per second
+    for p in (6, 7):
+        ctx.probe_ceiling(p, 4)
+        rate, ms = ctx.probe_ceiling(p, 16)
+        print(f"probe {p}: {rate / 1e12:.3f} TB/s ({ms:.3f} ms)", flush=True)
+        assert 5e11 < rate < 3e13
+    with pytest.raises(cra.CRError):
+        ctx.probe_ceiling(6, 65)
+
+
 def test_invalid_probe(ctx):
     with pytest.raises(cra.CRError):
-        ctx.probe_ceiling(6)
+        ctx.probe_ceiling(8)
     with pytest.raises(cra.CRError):
         ctx.probe_ceiling(0, 0)
