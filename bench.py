@@ -238,6 +238,63 @@ def measure_ceilings(ctx) -> dict:
                        "stream_write_bytes": r[6], "stream_read_bytes": r[7]}}
 
 
+def _warm(f):
+    """Runs f twice (the first call loads the code object and warms the
+    caches) and returns the second call's result."""
+    f()
+    return f()
+
+
+def run_kernels(ctx, ceil: dict) -> dict:
+    """The other hot kernels on their workloads, each checked for a complete
+    histogram, timed with HIP events on the library's stream after one
+    untimed call."""
+    import coding_reliability_amd as cra
+    out = {}
+    k = 32
+    h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_RANGE, 0, 1 << k))
+    assert sum(h) == 1 << k
+    out["hamming_exhaustive_39_32"] = _kernel_entry("hamming_exhaustive_39_32", 1 << k, ms, "Gwords/s", ceil,
+                                                    "(39,32) weights of all 2^32 data words")
+    h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 20, stride=65537))
+    assert sum(h) == 1 << 20
+    e = _kernel_entry("hamming_grid_39_32", 1 << 20, ms, "Gwords/s", ceil,
+                      "(39,32) integer grid d = i*65537 mod 2^32, i < 2^20")
+    # 2^20 words are well under a microsecond of device work: the launch is
+    # latency-bound, so its floor (the same kernel over 4096 words) is given
+    _, floor_ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 12, stride=65537))
+    e["launch_floor_ms"] = round(floor_ms, 4)
+    out["hamming_grid_39_32"] = e
+    S = 1 << 34
+    h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_MC, 0, S, seed=2026, stream=0))
+    assert sum(h) == S
+    out["hamming_mc_39_32"] = _kernel_entry("hamming_mc_39_32", S, ms, "Gwords/s", ceil, "(39,32) 2^34 Philox words")
+    S = 1 << 30
+    h, ms = _warm(lambda: ctx.hamming_weights(64, cra.HAM_MC, 0, S, seed=2026, stream=1))
+    assert sum(h) == S
+    out["hamming_mc_72_64"] = _kernel_entry("hamming_mc_72_64", S, ms, "Gwords/s", ceil, "(72,64) 2^30 Philox words")
+    h, ms = _warm(lambda: ctx.an_grid_ref(1, 32, 1001, 0, 61))
+    pairs = sum(h)
+    out["an_grid1d_ref"] = _kernel_entry("an_grid1d_ref", pairs, ms, "Gpairs/s", ceil,
+                                         f"reference grid 1D, A=61, k=32, m=1001 (38-bit codewords, {pairs} pairs)")
+    m = 1 << 16
+    h, ms = _warm(lambda: ctx.an_mc(61, 24, m, 16384))
+    assert sum(h) == (1 << 24) * m
+    out["an_mc"] = _kernel_entry("an_mc", (1 << 24) * m, ms, "Gpairs/s", ceil,
+                                 "AN Monte Carlo, A=61, k=24, 2^16 samples")
+    best, cands, ms = _warm(lambda: ctx.an_candidates(24, 16))
+    # pruning skips most (A, d) steps, so the work is counted as the steps the
+    # candidates (A attaining the maximum, never pruned) must run in full: a
+    # lower bound of the executed steps, hence a lower bound of the fraction
+    steps = len(cands) * ((1 << 24) - 1)
+    e = _kernel_entry("an_candidates", steps, ms, "Gsteps/s", ceil,
+                      "ANcandidates cell k=24, |A|=16 (16384 odd A); rate = steps of the A attaining the maximum "
+                      "(each runs all 2^24-1 steps) per second, a lower bound")
+    e["best"], e["candidates"] = best, len(cands)
+    out["an_candidates"] = e
+    return out
+
+
 # Timings the reference records in its own result files (distance_distribution/
 # results/...), one K80-class GPU unless stated: (seconds, source).
 REFERENCE_RECORDED = {

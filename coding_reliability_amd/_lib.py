@@ -129,12 +129,34 @@ SIGNATURES = {
 }
 
 
+def _one_hip_runtime() -> None:
+    """PyTorch wheels for ROCm carry their own HIP and HSA runtimes
+    (torch/lib/libamdhip64.so, soname libamdhip64.so.7), and a process can
+    bring up the GPU through one runtime instance only: whichever starts
+    second finds no device.  libcodrel.so needs libamdhip64.so.7, so when
+    torch is installed its runtime is loaded first (without importing torch);
+    libcodrel.so then binds to it by soname, and a later `import torch`
+    reuses the same file.  Device pointers and streams from torch and from
+    this library are then valid in each other's calls."""
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.origin:
+        return
+    hip = os.path.join(os.path.dirname(spec.origin), "lib", "libamdhip64.so")
+    if os.path.exists(hip):
+        C.CDLL(hip, mode=C.RTLD_GLOBAL)
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise CRError(3, f"{LIB_PATH} is missing; build it with "
                              "`python -m coding_reliability_amd.build` (no CPU fallback exists)")
+        _one_hip_runtime()
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name, None)

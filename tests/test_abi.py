@@ -72,3 +72,30 @@ def test_context_without_device_is_an_error():
         L.cr_context_destroy(h)
         pytest.skip("a device is visible")
     assert s in (3, 5) and L.cr_last_error()
+
+
+ONE_RUNTIME = r"""
+import coding_reliability_amd as cra
+from coding_reliability_amd._lib import lib
+lib()
+import torch
+paths = set()
+for line in open('/proc/self/maps'):
+    p = line.split()[-1]
+    if 'libamdhip64' in p or 'libhsa-runtime64' in p:
+        paths.add(p)
+print('\n'.join(sorted(paths)))
+"""
+
+
+def test_library_then_torch_share_one_hip_runtime():
+    # loading libcodrel.so before torch must not bring a second HIP/HSA
+    # runtime into the process (the second one to start finds no GPU)
+    import subprocess
+    import sys
+    pytest.importorskip("torch")
+    r = subprocess.run([sys.executable, "-c", ONE_RUNTIME], capture_output=True, text=True, cwd=ROOT, timeout=300)
+    assert r.returncode == 0, r.stderr
+    paths = r.stdout.split()
+    assert sum("libamdhip64" in p for p in paths) == 1, paths
+    assert sum("libhsa-runtime64" in p for p in paths) == 1, paths

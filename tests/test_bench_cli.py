@@ -1,5 +1,6 @@
 """bench.py's command-line contract on the CPU: more than one GPU is refused
-with a message and exit status 2 (this engine runs on one GPU)."""
+with a message and exit status 2 (this engine runs on one GPU), and every name the
+script reads is bound."""
 import json
 import os
 import subprocess
@@ -14,3 +15,25 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     msg = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in msg["error"]
+
+
+def test_every_name_bench_reads_is_bound():
+    # the GPU legs of bench.py only run on the GPU: a name that no statement
+    # binds (a deleted helper) must fail here, before a GPU run
+    import ast
+    import builtins
+    tree = ast.parse(open(os.path.join(ROOT, "bench.py")).read())
+    bound = set(dir(builtins)) | {"__file__"}
+    for n in ast.walk(tree):
+        if isinstance(n, (ast.FunctionDef, ast.ClassDef)):
+            bound.add(n.name)
+        elif isinstance(n, ast.Name) and isinstance(n.ctx, ast.Store):
+            bound.add(n.id)
+        elif isinstance(n, ast.arg):
+            bound.add(n.arg)
+        elif isinstance(n, (ast.Import, ast.ImportFrom)):
+            bound.update((a.asname or a.name).split(".")[0] for a in n.names)
+        elif isinstance(n, ast.ExceptHandler) and n.name:
+            bound.add(n.name)
+    read = {n.id for n in ast.walk(tree) if isinstance(n, ast.Name) and isinstance(n.ctx, ast.Load)}
+    assert read <= bound, sorted(read - bound)
