@@ -85,13 +85,15 @@ __device__ __forceinline__ uint64_t following_item(unsigned long long *counter, 
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): ten rounds of the 4x32
 // Philox S-box, multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key increments
 // 0x9E3779B9 / 0xBB67AE85.  c is the counter on input, the output on return.
+// A round is two 32x32->64 products (v_mad_u64_u32) and two three-input XORs
+// (v_bitop3_b32 with truth table 0x96, the key in a scalar register).
 __device__ __forceinline__ void philox4x32_10(uint32_t (&c)[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
         const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
-        const uint32_t x0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
-        const uint32_t x2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        const uint32_t x0 = __builtin_amdgcn_bitop3_b32((uint32_t)(p1 >> 32), c[1], k0, 0x96);
+        const uint32_t x2 = __builtin_amdgcn_bitop3_b32((uint32_t)(p0 >> 32), c[3], k1, 0x96);
         c[1] = (uint32_t)p1;
         c[3] = (uint32_t)p0;
         c[0] = x0;

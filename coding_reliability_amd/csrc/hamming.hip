@@ -29,13 +29,19 @@ using namespace dev;
 // path); small jobs use smaller items so that they still fill the device.
 
 // Syndrome tables: the syndrome (bit t = parity(d & m_t)) is linear over
-// GF(2), so it is the XOR of the syndromes of the data word's bytes; table j
-// holds the syndromes of the 256 values of byte j (4 tables for 32-bit data,
-// 8 for 64-bit).  The masks are truncated to k bits, so bits of d above k do
-// not change the syndrome.  The tables sit at the start of the wave's LDS,
-// the histogram rows after them.
+// GF(2), so it is the XOR of the syndromes of the data word's bit fields.
+// Each 32-bit half of the word is cut into fields of 7, 7, 7, 7 and 4 bits;
+// table f (f = 5h + i for field i of half h) holds the syndromes of the
+// field's 2^7 (or 2^4) values, one byte each, at byte offset 128 f: 5 tables
+// for 32-bit data, 10 for 64-bit.  A 128-byte table spans 32 consecutive
+// dwords, one in each LDS bank, so two lanes of a lookup only meet in a bank
+// on the same dword, whose read they share: the lookups are free of bank
+// conflicts (a 256-entry byte table puts two dwords in every bank).  The masks
+// are truncated to k bits, so bits of d above k do not change the syndrome.
+constexpr uint32_t kFields = 5;  // fields per 32-bit half
+
 template <bool W64>
-constexpr uint32_t kTabBytes = W64 ? 8 * 256 : 4 * 256;
+constexpr uint32_t kTabBytes = (W64 ? 2 : 1) * kFields * 128;
 
 template <bool W64>
 __device__ void build_tables(uint8_t *tab, const uint64_t *masks) {
@@ -43,48 +49,59 @@ __device__ void build_tables(uint8_t *tab, const uint64_t *masks) {
 #pragma unroll
     for (int t = 0; t < 7; ++t) m[t] = masks[t];
     for (uint32_t e = threadIdx.x; e < kTabBytes<W64>; e += 64) {
-        const uint64_t x = (uint64_t)(e & 255) << (8 * (e >> 8));
+        const uint32_t f = e >> 7, v = e & 127, h = f / kFields, i = f % kFields;
         uint32_t syn = 0;
+        if (i < kFields - 1 || v < 16) {
+            const uint64_t x = (uint64_t)v << (32 * h + 7 * i);
 #pragma unroll
-        for (int t = 0; t < 7; ++t) syn |= (uint32_t)(__popcll(x & m[t]) & 1) << t;
+            for (int t = 0; t < 7; ++t) syn |= (uint32_t)(__popcll(x & m[t]) & 1) << t;
+        }
         tab[e] = (uint8_t)syn;
     }
     __syncthreads();
 }
 
-// Weight of the non-extended codeword of data word d & dmask: popcount of the
-// data plus popcount of the syndrome.
-template <bool W64>
-__device__ __forceinline__ uint32_t weight_tab(uint64_t d, uint64_t dmask, const uint8_t *tab) {
-    const uint32_t lo = (uint32_t)d;
-    uint32_t syn = tab[lo & 255] ^ tab[256 + ((lo >> 8) & 255)] ^ tab[512 + ((lo >> 16) & 255)] ^ tab[768 + (lo >> 24)];
-    if (W64) {
-        const uint32_t hi = (uint32_t)(d >> 32);
-        syn ^= tab[1024 + (hi & 255)] ^ tab[1280 + ((hi >> 8) & 255)] ^ tab[1536 + ((hi >> 16) & 255)] ^
-               tab[1792 + (hi >> 24)];
-        return __popc(syn) + __popcll(d & dmask);
-    }
-    return __popc(syn) + __popc(lo & (uint32_t)dmask);
-}
-
-// Weights of U words at once: the byte-table reads of all U words are issued
-// before any of them is combined, so they are in flight together.
+// The table lookups of U words (all issued before any is combined, so they
+// are in flight together) and their combination into weights: popcount of the
+// data word plus popcount of the syndrome, the weight of the non-extended
+// codeword.
 template <bool W64, int U>
-__device__ __forceinline__ void weights_tab(const uint64_t (&d)[U], uint64_t dmask, const uint8_t *tab,
-                                            uint32_t (&w)[U]) {
-    constexpr int B = W64 ? 8 : 4;
-    uint32_t r[U][B];
+__device__ __forceinline__ void table_reads(const uint64_t (&d)[U], const uint8_t *tab,
+                                            uint32_t (&r)[U][(W64 ? 2 : 1) * kFields]) {
 #pragma unroll
     for (int u = 0; u < U; ++u)
 #pragma unroll
-        for (int b = 0; b < B; ++b) r[u][b] = tab[b * 256 + (uint32_t)((d[u] >> (8 * b)) & 255)];
+        for (uint32_t h = 0; h < (W64 ? 2u : 1u); ++h) {
+            const uint32_t x = (uint32_t)(d[u] >> (32 * h));
+            const uint8_t *t = tab + 128 * kFields * h;
+            r[u][kFields * h + 0] = t[x & 127];
+            r[u][kFields * h + 1] = t[128 + ((x >> 7) & 127)];
+            r[u][kFields * h + 2] = t[256 + ((x >> 14) & 127)];
+            r[u][kFields * h + 3] = t[384 + ((x >> 21) & 127)];
+            r[u][kFields * h + 4] = t[512 + (x >> 28)];
+        }
+}
+
+// FULL: dmask covers the whole word (k = 32 or 64), so the AND is left out.
+template <bool W64, int U, bool FULL = false>
+__device__ __forceinline__ void table_weights(const uint64_t (&d)[U], uint64_t dmask,
+                                              const uint32_t (&r)[U][(W64 ? 2 : 1) * kFields], uint32_t (&w)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
         uint32_t syn = r[u][0];
 #pragma unroll
-        for (int b = 1; b < B; ++b) syn ^= r[u][b];
-        w[u] = __popc(syn) + (W64 ? (uint32_t)__popcll(d[u] & dmask) : (uint32_t)__popc((uint32_t)d[u] & (uint32_t)dmask));
+        for (uint32_t f = 1; f < (W64 ? 2 : 1) * kFields; ++f) syn ^= r[u][f];
+        const uint64_t dm = FULL ? d[u] : d[u] & dmask;
+        w[u] = __popc(syn) + (W64 ? (uint32_t)__popcll(dm) : (uint32_t)__popc((uint32_t)dm));
     }
+}
+
+template <bool W64, int U>
+__device__ __forceinline__ void weights_tab(const uint64_t (&d)[U], uint64_t dmask, const uint8_t *tab,
+                                            uint32_t (&w)[U]) {
+    uint32_t r[U][(W64 ? 2 : 1) * kFields];
+    table_reads<W64, U>(d, tab, r);
+    table_weights<W64, U>(d, dmask, r, w);
 }
 
 // Syndrome of a data word: bit t = parity(x & m_t) (the seventh mask is zero
@@ -109,34 +126,75 @@ __device__ __forceinline__ uint32_t syndrome64(uint64_t x, const uint64_t (&m)[7
 // popc(x) are per-lane constants for ql < 16; S(dh), popc(dh) are wave-uniform.
 constexpr uint32_t kFastUnroll = 16;
 
-// One Monte Carlo item: lane l draws the Philox blocks of its samples (4
-// words per block for k <= 32, 2 for k > 32).  CHECK: the item reaches past
-// the job's sample count, so samples >= count go to the dummy row.
+// One Monte Carlo item: lane l takes the Philox blocks B0 + 64q + l, q <
+// spl/PER, B0 = (first + base)/PER (PER = 4 words per block for k <= 32, 2
+// for k > 32: sample s of the job is word s mod PER of block s/PER).  Three
+// blocks are in flight: at step q the table lookups of block q+1 are issued,
+// then the Philox rounds of block q+2 run interleaved with the combination
+// of block q, whose lookups were issued a step earlier.  The last two steps
+// generate and look up blocks past the item's end, which are not counted.
+// CHECK: the item reaches past the job's sample count, so samples >= count
+// go to the dummy row.
+template <uint32_t PER>
+__device__ __forceinline__ void mc_words(const uint32_t (&c)[4], uint64_t (&x)[PER]) {
+#pragma unroll
+    for (uint32_t u = 0; u < PER; ++u)
+        x[u] = PER == 4 ? (uint64_t)c[u] : ((uint64_t)c[2 * u] | ((uint64_t)c[2 * u + 1] << 32));
+}
+
+__device__ __forceinline__ void mc_block(uint32_t (&c)[4], uint64_t b, uint32_t stream, uint32_t k0, uint32_t k1) {
+    c[0] = (uint32_t)b;
+    c[1] = (uint32_t)(b >> 32);
+    c[2] = stream;
+    c[3] = kMcTag;
+    philox4x32_10(c, k0, k1);
+}
+
+template <bool W64, uint32_t PER, bool CHECK, bool FULL>
+__device__ __forceinline__ void mc_blocks(uint32_t *lane_col, const uint8_t *tab, const HamJob &job, uint64_t base,
+                                          uint32_t spl, uint64_t count, uint64_t dmask, uint32_t dummy) {
+    constexpr uint32_t F = (W64 ? 2 : 1) * kFields;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t k0 = (uint32_t)job.seed, k1 = (uint32_t)(job.seed >> 32);
+    const uint32_t nq = spl / PER;
+    const uint64_t b0 = (job.first + base) / PER + lane;
+    uint32_t c[4];
+    uint64_t x[2][PER];
+    uint32_t r[2][PER][F];
+    mc_block(c, b0, job.stream, k0, k1);
+    mc_words<PER>(c, x[0]);
+    table_reads<W64, PER>(x[0], tab, r[0]);
+    mc_block(c, b0 + 64, job.stream, k0, k1);
+    // nq is even (spl is a power of two >= 16): two steps per iteration, the
+    // buffers of blocks q and q+1 trading places
+    for (uint32_t q = 0; q < nq; q += 2) {
+#pragma unroll
+        for (uint32_t h = 0; h < 2; ++h) {
+            mc_words<PER>(c, x[h ^ 1]);
+            table_reads<W64, PER>(x[h ^ 1], tab, r[h ^ 1]);
+            mc_block(c, b0 + 64ull * (q + h + 2), job.stream, k0, k1);
+            uint32_t w[PER];
+            table_weights<W64, PER, FULL>(x[h], dmask, r[h], w);
+            const uint64_t s = base + ((uint64_t)(q + h) * 64 + lane) * PER;  // first local sample of this lane
+#pragma unroll
+            for (uint32_t u = 0; u < PER; ++u)
+                lds_inc(lane_col + ((!CHECK || s + u < count) ? w[u] : dummy) * kStride);
+        }
+    }
+}
+
 template <bool W64, bool CHECK>
 __device__ __forceinline__ void mc_item(uint32_t *lane_col, const uint8_t *tab, const HamJob &job, uint64_t base,
                                         uint32_t spl, uint64_t count, uint64_t dmask, uint32_t dummy) {
-    const uint32_t lane = threadIdx.x;
-    const uint32_t per = job.k <= 32 ? 4 : 2;  // words per Philox block
-    const uint32_t k0 = (uint32_t)job.seed, k1 = (uint32_t)(job.seed >> 32);
-    for (uint32_t q = 0; q < spl / per; ++q) {
-        const uint64_t s = base + ((uint64_t)q * 64 + lane) * per;  // first local sample of this lane
-        const uint64_t blk = (job.first + s) / per;
-        uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), job.stream, kMcTag};
-        philox4x32_10(c, k0, k1);
-        // the block's table reads are all issued before its increments
-        if (per == 4) {
-            const uint64_t x[4] = {c[0], c[1], c[2], c[3]};
-            uint32_t w[4];
-            weights_tab<W64, 4>(x, dmask, tab, w);
-#pragma unroll
-            for (uint32_t u = 0; u < 4; ++u) lds_inc(lane_col + ((!CHECK || s + u < count) ? w[u] : dummy) * kStride);
-        } else {
-            const uint64_t x[2] = {(uint64_t)c[0] | ((uint64_t)c[1] << 32), (uint64_t)c[2] | ((uint64_t)c[3] << 32)};
-            uint32_t w[2];
-            weights_tab<W64, 2>(x, dmask, tab, w);
-#pragma unroll
-            for (uint32_t u = 0; u < 2; ++u) lds_inc(lane_col + ((!CHECK || s + u < count) ? w[u] : dummy) * kStride);
-        }
+    if (!W64 || job.k <= 32) {
+        if (job.k == 32)
+            mc_blocks<W64, 4, CHECK, true>(lane_col, tab, job, base, spl, count, dmask, dummy);
+        else
+            mc_blocks<W64, 4, CHECK, false>(lane_col, tab, job, base, spl, count, dmask, dummy);
+    } else if (job.k == 64) {
+        mc_blocks<W64, 2, CHECK, true>(lane_col, tab, job, base, spl, count, dmask, dummy);
+    } else {
+        mc_blocks<W64, 2, CHECK, false>(lane_col, tab, job, base, spl, count, dmask, dummy);
     }
 }
 
@@ -146,9 +204,11 @@ __device__ __forceinline__ void mc_item(uint32_t *lane_col, const uint8_t *tab, 
 // k <= 32: only the low k bits reach the weight), or list[base + 64q + l].
 // kSeqUnroll steps per iteration, all table reads (and list loads) of the
 // batch issued before its increments, so the reads overlap instead of each
-// waiting behind the previous increment.  spl is a multiple of kSeqUnroll.
+// waiting behind the previous increment (8 steps of 5 lookups for 32-bit
+// words, 4 of 10 for 64-bit).  spl is a multiple of kSeqUnroll.
 // CHECK: steps past the job's count go to the dummy row.
-constexpr uint32_t kSeqUnroll = 8;
+template <bool W64>
+constexpr uint32_t kSeqUnroll = W64 ? 4 : 8;
 
 template <bool W64, bool CHECK, bool LIST>
 __device__ __forceinline__ void seq_item(uint32_t *lane_col, const uint8_t *tab, const HamJob &job, uint64_t base,
@@ -161,23 +221,24 @@ __device__ __forceinline__ void seq_item(uint32_t *lane_col, const uint8_t *tab,
     word_t d = (word_t)(job.first + (base + lane) * stride);
     const word_t step = (word_t)(64 * stride);
     const uint64_t *list = LIST ? job.list + base + lane : nullptr;
-    for (uint32_t q = 0; q < spl; q += kSeqUnroll) {
-        uint64_t x[kSeqUnroll];
-        uint32_t w[kSeqUnroll];
+    constexpr uint32_t U = kSeqUnroll<W64>;
+    for (uint32_t q = 0; q < spl; q += U) {
+        uint64_t x[U];
+        uint32_t w[U];
         if (LIST) {
 #pragma unroll
-            for (uint32_t u = 0; u < kSeqUnroll; ++u)
+            for (uint32_t u = 0; u < U; ++u)
                 x[u] = (!CHECK || (q + u) * 64 + lane < left) ? list[(q + u) * 64] : 0;
         } else {
 #pragma unroll
-            for (uint32_t u = 0; u < kSeqUnroll; ++u) {
+            for (uint32_t u = 0; u < U; ++u) {
                 x[u] = (uint64_t)d;
                 d += step;
             }
         }
-        weights_tab<W64, kSeqUnroll>(x, dmask, tab, w);
+        weights_tab<W64, U>(x, dmask, tab, w);
 #pragma unroll
-        for (uint32_t u = 0; u < kSeqUnroll; ++u)
+        for (uint32_t u = 0; u < U; ++u)
             lds_inc(lane_col + ((!CHECK || (q + u) * 64 + lane < left) ? w[u] : dummy) * kStride);
     }
 }
@@ -199,17 +260,16 @@ __device__ __forceinline__ uint32_t find_job(const HamJob *jobs, uint32_t njobs,
 template <bool W64>
 __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs, uint32_t njobs, uint64_t total_items,
                                                  unsigned long long *__restrict__ counter, uint32_t lds_rows) {
-    extern __shared__ uint32_t lds_all[];
-    uint8_t *const tab = reinterpret_cast<uint8_t *>(lds_all);
-    uint32_t *const lds = lds_all + kTabBytes<W64> / 4;
+    // the tables at a fixed LDS address (their offsets fold into the reads),
+    // the histogram rows in the dynamic LDS after them
+    __shared__ uint8_t tab[kTabBytes<W64>];
+    extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
     lds_clear(lds, lds_rows);
     uint32_t *const lane_col = lds + lane;
     const uint32_t dummy = lds_rows - 1;
     uint32_t cur = 0xFFFFFFFFu;
     uint64_t pending = 0;
-    uint32_t fast_job = 0xFFFFFFFFu;  // job whose per-lane constants are loaded
-    uint32_t fsyn[kFastUnroll], fbase[kFastUnroll];
 
     for (uint64_t t = first_item(); t < total_items; t = following_item(counter, t)) {
         const uint32_t jb = find_job(jobs, njobs, t, cur);
@@ -240,14 +300,15 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
             // full item of consecutive words (exhaustive) whose first word is a
             // multiple of the item size 64*spl, a power of two: the item's
             // words are d0 | x for x < 64*spl (disjoint bits)
-            if (fast_job != jb) {
+            // per-lane constants of the item, S(x) from the tables (x < 2^10
+            // spans fields 0 and 1); kept to this path so that they hold no
+            // registers in the others
+            uint32_t fsyn[kFastUnroll], fbase[kFastUnroll];
 #pragma unroll
-                for (uint32_t ql = 0; ql < kFastUnroll; ++ql) {
-                    const uint32_t x = (ql << 6) | lane;
-                    fsyn[ql] = syndrome32(x, m);
-                    fbase[ql] = lane * 4 + ((uint32_t)__popc(x) << 8);  // byte offset in the LDS rows
-                }
-                fast_job = jb;
+            for (uint32_t ql = 0; ql < kFastUnroll; ++ql) {
+                const uint32_t x = (ql << 6) | lane;
+                fsyn[ql] = tab[x & 127] ^ tab[128 + (x >> 7)];
+                fbase[ql] = lane * 4 + ((uint32_t)__popc(x) << 8);  // byte offset in the LDS rows
             }
             const uint64_t d0 = (job.first + base) & dmask;
             for (uint32_t qh = 0; qh < spl / kFastUnroll; ++qh) {
@@ -385,10 +446,9 @@ __global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint6
     if (dirty) flush_rows(lds, 0, nbins, out);
 }
 
-// Dynamic LDS of a Hamming wave: syndrome tables, then the histogram rows.
-size_t ham_lds_bytes(bool w64, uint32_t lds_rows) {
-    return (w64 ? kTabBytes<true> : kTabBytes<false>) + (size_t)lds_rows * kStride * sizeof(uint32_t);
-}
+// Dynamic LDS of a Hamming wave: the histogram rows (the syndrome tables are
+// the kernel's static LDS).
+size_t ham_lds_bytes(bool, uint32_t lds_rows) { return (size_t)lds_rows * kStride * sizeof(uint32_t); }
 
 }  // namespace
 

@@ -71,21 +71,25 @@ extern "C" cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *r
     // 40-150 us per launch.  Launches estimated below 100 us of work
     // therefore run with a static assignment.  Measured rates (words/s):
     // exhaustive 7.5e12, 32-bit range/stride 1.9e12, 64-bit 0.55e12, Monte
-    // Carlo 1.4e12 / 0.57e12, lists about half the stride rates.
+    // Carlo 1.55e12 / 0.75e12, lists about half the stride rates.
     double rate = req->k > 32 ? 0.55e12 : 1.9e12;
     if (req->source == CR_HAM_RANGE && req->first == 0) rate = 7.5e12;
-    if (req->source == CR_HAM_MC) rate = req->k > 32 ? 0.57e12 : 1.4e12;
+    if (req->source == CR_HAM_MC) rate = req->k > 32 ? 0.75e12 : 1.55e12;
     if (req->source == CR_HAM_LIST) rate *= 0.5;
     const bool short_launch = (double)req->count / rate < 100e-6;
     {
-        // items of 64 x spl words, spl a power of two in [16, kHamMaxLaneSamples]:
-        // about 8 or more per resident wave for a static assignment (a fine
-        // balance), 4 or more for the queue
+        // items of 64 x spl words, spl the smallest power of two in [16,
+        // kHamMaxLaneSamples] that gives at most 8 items per resident wave for
+        // a static assignment or 16 for the queue (the last items set the
+        // launch's tail: with at most 4 per wave a 2^30-word (72,64) Monte
+        // Carlo launch ran 20 % below the rate of 2^34 words), and for the
+        // queue at least rate/8e7 words per item (the counter's 8e7 items/s)
         const uint64_t waves = (uint64_t)hamming_waves_per_cu(need + 1) * ctx->cus;
-        const uint64_t per_wave = short_launch ? 8 : 4;
+        const uint64_t per_wave = short_launch ? 8 : 16;
         const uint64_t want = (req->count + 64 * per_wave * waves - 1) / (64 * per_wave * waves);
+        const double min_words = short_launch ? 0.0 : rate / 8e7;
         uint64_t spl = 16;
-        while (spl < want && spl < kHamMaxLaneSamples) spl *= 2;
+        while ((spl < want || 64.0 * (double)spl < min_words) && spl < kHamMaxLaneSamples) spl *= 2;
         job.lane_samples = (uint32_t)spl;
     }
     job.items = hamming_items(req->count, job.lane_samples);

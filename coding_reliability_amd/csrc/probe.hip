@@ -97,7 +97,7 @@ __global__ __launch_bounds__(64) void probe_lds(uint32_t iters, uint32_t seed, u
 
 // Philox blocks whose words each go through the AN Monte Carlo pair work
 // (24-bit multiply, xor, popcount-accumulate): the mix of the Monte Carlo
-// kernels without their LDS increments; 60 + 4 x 3 VALU per block.
+// kernels without their LDS increments; 40 + 4 x 3 VALU per block.
 __global__ __launch_bounds__(64) void probe_mixed(uint32_t iters, uint32_t seed, uint32_t *out) {
     const uint32_t lane = threadIdx.x;
     uint32_t c[2][4];
@@ -235,8 +235,8 @@ extern "C" cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t 
     CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     // units per lane and loop iteration (VALU instructions of the loop body, or
     // LDS increments)
-    const double per_lane_iter[kProbeCount - 2] = {2.0 * kValuChains, 4 * 60.0, (double)kLdsRows,
-                                                   2 * (60.0 + 4 * 3), 32.0, (double)kLdsRows};
+    const double per_lane_iter[kProbeCount - 2] = {2.0 * kValuChains, 4 * 40.0, (double)kLdsRows,
+                                                   2 * (40.0 + 4 * 3), 32.0, (double)kLdsRows};
     const double units = (double)waves * 64.0 * (double)iters * per_lane_iter[probe];
     *units_per_s = ms > 0 ? units / (ms * 1e-3) : 0;
     if (kernel_ms) *kernel_ms = ms;
