@@ -170,52 +170,52 @@ def run_batched(ctx, autotune: bool = True) -> dict:
     return out
 
 
-# The other kernels are measured against the same two ceilings as the pair
-# kernel: the VALU issue ceiling and the LDS increment ceiling, both measured
-# in this run (measure_ceilings).  Per unit of work (word, pair, step) each
 # kernel executes `valu` VALU instructions -- the SQ_INSTS_VALU counts of its
-# profile under profiles/, divided by the units of the launch -- and `lds`
-# histogram increments (wave-wide ds_add_u32 lanes).  frac_valu is the VALU
-# issue utilisation (executed instructions/s over the ceiling), frac_lds the
-# LDS increment rate over its ceiling; frac is the larger (the two pipelines
-# run concurrently), named by "bound".  `ops` is the algorithmic int32 work
-# per unit for reference: xor + popcount + increment per enumerated word or
-# pair (2W+1 with the high dword folded), 4 byte lookups + 3 xors + 2
-# popcounts per sampled 32-bit word (8 + 7 + 2 for 64-bit), 60 per Philox
-# block (15 per 32-bit Monte Carlo word, 30 per 64-bit), 9 per ANcandidates
-# step.  The sampled Hamming kernels also read syndrome tables in LDS (4 or 8
-# ds_read_u8 per word), not counted in frac_lds.
+# profile under profiles/, divided by the units of the launch --, `lds`
+# histogram increments (wave-wide ds_add_u32 lanes) and `reads` syndrome-table
+# lookups (ds_read_u8 lanes).  frac_valu is the VALU issue utilisation
+# (executed instructions/s over the ceiling), frac_lds the share of time the
+# LDS is busy (increments over the increment ceiling plus lookups over the
+# table-read ceiling); frac is the larger (the two pipelines run
+# concurrently), named by "bound".  `ops` is the algorithmic int32 work per
+# unit for reference: xor + popcount + increment per enumerated word or pair
+# (2W+1 with the high dword folded), per sampled 32-bit word 5 field lookups
+# + 4 xors + 2 popcounts + 1 increment (10 + 9 + 3 + 1 for 64-bit), 40 per
+# Philox block (10 per 32-bit Monte Carlo word, 20 per 64-bit), 9 per
+# ANcandidates step.
 KERNEL_WORK = {
-    # name: (executed VALU per unit, LDS increments per unit, algorithmic ops per unit, profile)
-    "hamming_exhaustive_39_32": (3.09, 1, 3, "profiles/ham_kernel_39_32_exhaustive_grid_mc_and_72_64_mc.md"),
-    "hamming_grid_39_32": (None, 1, 9, "latency-bound launch"),
-    "hamming_mc_39_32": (26.53, 1, 24, "profiles/ham_kernel_39_32_exhaustive_grid_mc_and_72_64_mc.md"),
-    "hamming_mc_72_64": (51.27, 1, 46, "profiles/ham_kernel_39_32_exhaustive_grid_mc_and_72_64_mc.md"),
-    "an_grid1d_ref": (3.01, 1, 3, "profiles/rect_kernel_grid1d_A61_k32_m1001.md"),
-    "an_mc": (18.02, 1, 19, "profiles/an_mc_kernel_A61_k24_m65536.md"),
-    "an_candidates": (None, 0, 9, "lower bound: algorithmic steps of the maximising A"),
+    # name: (executed VALU per unit, LDS increments per unit, LDS table reads per unit,
+    #        algorithmic ops per unit, profile)
+    "hamming_exhaustive_39_32": (3.07, 1, 0, 3, "profiles/ham_kernel_39_32_exhaustive_grid_mc_and_72_64_mc.md"),
+    "hamming_grid_39_32": (None, 1, 5, 12, "latency-bound launch"),
+    "hamming_mc_39_32": (21.92, 1, 5, 22, "profiles/ham_kernel_39_32_exhaustive_grid_mc_and_72_64_mc.md"),
+    "hamming_mc_72_64": (42.47, 1, 10, 43, "profiles/ham_kernel_39_32_exhaustive_grid_mc_and_72_64_mc.md"),
+    "an_grid1d_ref": (3.01, 1, 0, 3, "profiles/rect_kernel_grid1d_A61_k32_m1001.md"),
+    "an_mc": (16.02, 1, 0, 14, "profiles/an_mc_kernel_A61_k24_m65536.md"),
+    "an_candidates": (None, 0, 0, 9, "lower bound: algorithmic steps of the maximising A"),
 }
 
 
 def _kernel_entry(name: str, units: float, ms: float, unit: str, ceil: dict, workload: str) -> dict:
-    valu, lds, ops, src = KERNEL_WORK[name]
+    valu, lds, reads, ops, src = KERNEL_WORK[name]
     rate = units / (ms / 1e3)
     f_valu = rate * (valu if valu is not None else ops) / ceil["valu"]
-    f_lds = rate * lds / ceil["lds"] if lds else 0.0
-    bound = "valu" if f_valu >= f_lds else "lds_atomic"
+    f_lds = rate * lds / ceil["lds"] + rate * reads / ceil["lds_read"]
+    bound = "valu" if f_valu >= f_lds else "lds"
     return {"workload": workload, "kernel_ms": round(ms, 3), "rate": round(rate / 1e9, 3), "rate_unit": unit,
             "valu_per_unit": valu if valu is not None else ops, "valu_source": src if valu is not None else
-            "algorithmic ops (" + src + ")", "lds_increments_per_unit": lds, "ops_per_unit": ops,
-            "achieved_tops": round(rate * ops / 1e12, 3),
+            "algorithmic ops (" + src + ")", "lds_increments_per_unit": lds, "lds_table_reads_per_unit": reads,
+            "ops_per_unit": ops, "achieved_tops": round(rate * ops / 1e12, 3),
             "frac_valu": round(f_valu, 4), "frac_lds": round(f_lds, 4), "bound": bound,
             "frac": round(max(f_valu, f_lds), 4)}
 
 
 def _ceilings_json(ceil: dict) -> dict:
     return {"valu": round(ceil["valu"] / 1e12, 4), "lds": round(ceil["lds"] / 1e12, 4),
+            "lds_read": round(ceil["lds_read"] / 1e12, 4),
             "write": round(ceil["write"] / 1e12, 4), "read": round(ceil["read"] / 1e12, 4),
-            "unit": "T/s: VALU lane-instructions, LDS lane increments, TB/s streaming stores and loads over 256 MiB "
-                    "(cr_probe_ceiling, this run)",
+            "unit": "T/s: VALU lane-instructions, LDS lane increments, LDS lane-reads from 128-byte tables, TB/s "
+                    "streaming stores and loads over 256 MiB (cr_probe_ceiling, this run)",
             "probes": {kk: round(v / 1e12, 4) for kk, v in ceil["probes"].items()}}
 
 
@@ -223,19 +223,21 @@ def measure_ceilings(ctx) -> dict:
     """The device ceilings of this run (cr_probe_ceiling), each the best of
     three launches after one untimed one: VALU issue (the best of the
     xor+popcount, Philox and mixed probes, in lane-instructions/s), LDS
-    increments (lane increments/s, the better of the two lane patterns) and
-    streaming stores and loads over a 256 MiB buffer (bytes/s)."""
-    iters = {0: 1 << 15, 1: 1 << 12, 2: 1 << 14, 3: 1 << 12, 4: 1 << 14, 5: 1 << 14, 6: 16, 7: 16}
+    increments (lane increments/s, the better of the two lane patterns), LDS
+    reads from 128-byte tables (lane-reads/s) and streaming stores and loads
+    over a 256 MiB buffer (bytes/s)."""
+    iters = {0: 1 << 15, 1: 1 << 12, 2: 1 << 14, 3: 1 << 12, 4: 1 << 14, 5: 1 << 14, 6: 16, 7: 16, 8: 1 << 13}
 
     def best(probe):
         ctx.probe_ceiling(probe, iters[probe])
         return max(ctx.probe_ceiling(probe, iters[probe])[0] for _ in range(3))
 
-    r = {p: best(p) for p in range(8)}
-    return {"valu": max(r[0], r[1], r[3], r[4]), "lds": max(r[2], r[5]), "write": r[6], "read": r[7],
+    r = {p: best(p) for p in range(9)}
+    return {"valu": max(r[0], r[1], r[3], r[4]), "lds": max(r[2], r[5]), "lds_read": r[8], "write": r[6],
+            "read": r[7],
             "probes": {"valu_xor_popcount_8": r[0], "valu_philox": r[1], "lds_scattered_rows": r[2],
                        "valu_philox_pair_mix": r[3], "valu_xor_popcount_16": r[4], "lds_same_row": r[5],
-                       "stream_write_bytes": r[6], "stream_read_bytes": r[7]}}
+                       "stream_write_bytes": r[6], "stream_read_bytes": r[7], "lds_table_reads": r[8]}}
 
 
 def _warm(f):

@@ -130,12 +130,14 @@ class Context:
         check(lib().cr_device_memory(self._h, C.byref(f), C.byref(t)))
         return f.value, t.value
 
-    PROBE_VALU, PROBE_PHILOX, PROBE_LDS, PROBE_MIXED, PROBE_VALU16, PROBE_LDS_ROW, PROBE_WRITE, PROBE_READ = range(8)
+    (PROBE_VALU, PROBE_PHILOX, PROBE_LDS, PROBE_MIXED, PROBE_VALU16, PROBE_LDS_ROW, PROBE_WRITE, PROBE_READ,
+     PROBE_LDS_TABLE) = range(9)
 
     def probe_ceiling(self, probe: int, iters: int = 1 << 14) -> tuple[float, float]:
         """(units per second, device ms) of a ceiling probe (cr_probe_ceiling):
-        VALU lane-instructions (probes 0, 1, 3, 4), LDS lane increments (2, 5) or
-        bytes (6, 7: streaming stores / loads over 256 MiB, iters passes)."""
+        VALU lane-instructions (probes 0, 1, 3, 4), LDS lane increments (2, 5),
+        bytes (6, 7: streaming stores / loads over 256 MiB, iters passes) or LDS
+        lane-reads from 128-byte tables (8)."""
         rate, ms = C.c_double(0), C.c_double(0)
         check(lib().cr_probe_ceiling(self._h, probe, iters, C.byref(rate), C.byref(ms)))
         return rate.value, ms.value

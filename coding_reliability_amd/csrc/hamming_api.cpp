@@ -85,7 +85,7 @@ extern "C" cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *r
         // Carlo launch ran 20 % below the rate of 2^34 words), and for the
         // queue at least rate/8e7 words per item (the counter's 8e7 items/s)
         const uint64_t waves = (uint64_t)hamming_waves_per_cu(need + 1) * ctx->cus;
-        const uint64_t per_wave = short_launch ? 8 : 16;
+        const uint64_t per_wave = short_launch ? 8 : 32;
         const uint64_t want = (req->count + 64 * per_wave * waves - 1) / (64 * per_wave * waves);
         const double min_words = short_launch ? 0.0 : rate / 8e7;
         uint64_t spl = 16;

@@ -8,16 +8,19 @@
 //                   8 independent chains: the pair, grid and exhaustive
 //                   Hamming inner loops; 16 per iteration
 //   1 kProbePhilox  Philox4x32-10 blocks on 4 independent streams per lane
-//                   (20 v_mad_u64_u32 + 40 v_xor_b32 per block)
+//                   (20 v_mad_u64_u32 + 20 v_bitop3_b32 per block)
 //   2 kProbeLds     wave-wide ds_add_u32 to lane-private LDS counters in the
 //                   histogram layout of every kernel (row stride 64 dwords),
 //                   the lanes of an instruction in different rows
 //   3 kProbeMixed   Philox blocks whose words go through the AN Monte Carlo
-//                   pair work (multiply, xor, popcount): 72 per block
+//                   pair work (multiply, xor, popcount): 52 per block
 //   4 kProbeValu16  as 0 on 16 chains
 //   5 kProbeLdsRow  as 2, all lanes of an instruction in one row
 //   6 kProbeWrite   streaming 16-byte stores over a 256 MiB buffer (bytes/s)
 //   7 kProbeRead    streaming 16-byte loads over the same buffer (bytes/s)
+//   8 kProbeLdsTab  ds_read_u8 from 128-byte tables (one dword per LDS bank,
+//                   the Hamming syndrome tables' layout), 8 tables read at one
+//                   address and combined by three-input XORs: LDS lane-reads/s
 #include "cr_device.h"
 #include "cr_internal.h"
 
@@ -27,7 +30,7 @@ namespace {
 using namespace dev;
 
 constexpr uint32_t kProbeValu = 0, kProbePhilox = 1, kProbeLds = 2, kProbeMixed = 3, kProbeValu16 = 4,
-                   kProbeLdsRow = 5, kProbeWrite = 6, kProbeRead = 7, kProbeCount = 8;
+                   kProbeLdsRow = 5, kProbeWrite = 6, kProbeRead = 7, kProbeLdsTab = 8, kProbeCount = 9;
 constexpr size_t kStreamBytes = 256ull << 20;
 constexpr int kValuChains = 8;
 constexpr int kLdsRows = 16;
@@ -163,6 +166,40 @@ __global__ __launch_bounds__(64) void probe_lds_row(uint32_t iters, uint32_t see
 }
 
 // Streaming stores / loads: `iters` passes over n uint4 with a grid stride.
+// Byte lookups in the Hamming syndrome tables' layout: 8 tables of 128 bytes
+// (32 dwords each, one per LDS bank), read at one per-lane index on 4
+// independent chains; the next index depends on the 8 bytes read (three
+// v_bitop3_b32 and one multiply-xor per 8 reads, so the LDS and not the VALU
+// sets the rate).
+constexpr int kTabChains = 4;
+__global__ __launch_bounds__(64) void probe_lds_tab(uint32_t iters, uint32_t seed, uint32_t *out) {
+    __shared__ uint32_t tab[8 * 32];
+    const uint32_t lane = threadIdx.x;
+    for (uint32_t i = lane; i < 8 * 32; i += 64) tab[i] = (i + seed) * 2654435761u;
+    __syncthreads();
+    const uint8_t *tb = reinterpret_cast<const uint8_t *>(tab);
+    uint32_t x[kTabChains];
+#pragma unroll
+    for (int j = 0; j < kTabChains; ++j) x[j] = (lane * 7919u + j * 104729u + seed + blockIdx.x) * 2654435761u;
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int j = 0; j < kTabChains; ++j) {
+            const uint32_t idx = x[j] & 127;
+            uint32_t v[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) v[t] = tb[128 * t + idx];
+            uint32_t acc = __builtin_amdgcn_bitop3_b32(v[0], v[1], v[2], 0x96);
+            acc = __builtin_amdgcn_bitop3_b32(acc, v[3], v[4], 0x96);
+            acc = __builtin_amdgcn_bitop3_b32(acc, v[5], v[6], 0x96);
+            x[j] = __builtin_amdgcn_bitop3_b32(acc, v[7], x[j] * 0x01000193u, 0x96);
+        }
+    }
+    uint32_t a = 0;
+#pragma unroll
+    for (int j = 0; j < kTabChains; ++j) a ^= x[j];
+    out[blockIdx.x * 64 + lane] = a;
+}
+
 __global__ __launch_bounds__(256) void probe_write(uint32_t iters, uint32_t seed, uint4 *buf, uint64_t n) {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint4 x = make_uint4(seed, threadIdx.x, blockIdx.x, 0);
@@ -214,10 +251,12 @@ extern "C" cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t 
         if (kernel_ms) *kernel_ms = ms;
         return CR_OK;
     }
-    const void *kernels[kProbeCount - 2] = {
+    const void *kernels[kProbeCount] = {
         reinterpret_cast<const void *>(&probe_valu),   reinterpret_cast<const void *>(&probe_philox),
         reinterpret_cast<const void *>(&probe_lds),    reinterpret_cast<const void *>(&probe_mixed),
-        reinterpret_cast<const void *>(&probe_valu16), reinterpret_cast<const void *>(&probe_lds_row)};
+        reinterpret_cast<const void *>(&probe_valu16), reinterpret_cast<const void *>(&probe_lds_row),
+        nullptr,                                       nullptr,
+        reinterpret_cast<const void *>(&probe_lds_tab)};
     const void *k = kernels[probe];
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, 0) != hipSuccess || per_cu <= 0)
@@ -233,10 +272,11 @@ extern "C" cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t 
     CR_HIP(hipStreamSynchronize(ctx->stream));
     float ms = 0;
     CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    // units per lane and loop iteration (VALU instructions of the loop body, or
-    // LDS increments)
-    const double per_lane_iter[kProbeCount - 2] = {2.0 * kValuChains, 4 * 40.0, (double)kLdsRows,
-                                                   2 * (40.0 + 4 * 3), 32.0, (double)kLdsRows};
+    // units per lane and loop iteration (VALU instructions of the loop body,
+    // LDS increments or LDS reads)
+    const double per_lane_iter[kProbeCount] = {2.0 * kValuChains, 4 * 40.0, (double)kLdsRows,
+                                               2 * (40.0 + 4 * 3), 32.0, (double)kLdsRows, 0.0, 0.0,
+                                               8.0 * kTabChains};
     const double units = (double)waves * 64.0 * (double)iters * per_lane_iter[probe];
     *units_per_s = ms > 0 ? units / (ms * 1e-3) : 0;
     if (kernel_ms) *kernel_ms = ms;

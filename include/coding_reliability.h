@@ -73,7 +73,8 @@ cr_status cr_device_memory(cr_context *ctx, uint64_t *free_bytes, uint64_t *tota
  *               16 independent chains (the pair, grid and exhaustive Hamming
  *               inner loops);
  *   probe 1:    VALU lane-instructions/s of Philox4x32-10 (the Monte Carlo
- *               generator: 20 64-bit multiplies and 40 xors per block);
+ *               generator: 20 64-bit multiplies and 20 three-input xors per
+ *               block);
  *   probe 3:    the same for Philox blocks whose words go through the AN
  *               Monte Carlo pair work (multiply, xor, popcount);
  *   probe 2, 5: lane increments/s of wave-wide ds_add_u32 to lane-private LDS
@@ -81,7 +82,9 @@ cr_status cr_device_memory(cr_context *ctx, uint64_t *free_bytes, uint64_t *tota
  *               of an instruction in different rows (2) or one row (5);
  *   probe 6, 7: bytes/s of streaming 16-byte stores (6) or loads (7), `iters`
  *               passes (at most 64) over a 256 MiB buffer, the size of the
- *               spectral path's batches.
+ *               spectral path's batches;
+ *   probe 8:    lane-reads/s of ds_read_u8 from 128-byte LDS tables (one
+ *               dword per bank), the Hamming kernels' syndrome lookups.
  * kernel_ms (may be NULL) receives the probe's device time. */
 cr_status cr_probe_ceiling(cr_context *ctx, uint32_t probe, uint32_t iters, double *units_per_s, double *kernel_ms);
 

@@ -17,7 +17,7 @@ def ctx():
 
 @pytest.fixture(scope="module")
 def ceilings(ctx):
-    iters = {0: 1 << 14, 1: 1 << 11, 2: 1 << 13, 3: 1 << 11, 4: 1 << 13, 5: 1 << 13}
+    iters = {0: 1 << 14, 1: 1 << 11, 2: 1 << 13, 3: 1 << 11, 4: 1 << 13, 5: 1 << 13, 8: 1 << 12}
     out = {}
     for p, it in iters.items():
         ctx.probe_ceiling(p, it)
@@ -48,6 +48,19 @@ def test_pair_kernel_within_ceilings(ctx, ceilings):
     assert 3 * rate <= 1.03 * valu
 
 
+def test_hamming_mc_within_lds_ceilings(ctx, ceilings):
+    # (72,64) Monte Carlo: 10 table reads and one increment per word share the
+    # LDS, so the time fractions of the two LDS ceilings add up to at most 1
+    S = 1 << 28
+    ctx.hamming_weights(64, cra.HAM_MC, 0, S, seed=3, stream=1)
+    h, ms = ctx.hamming_weights(64, cra.HAM_MC, 0, S, seed=3, stream=1)
+    assert sum(h) == S
+    rate = S / (ms / 1e3)
+    busy = rate * 10 / ceilings[8] + rate / max(ceilings[2], ceilings[5])
+    print(f"(72,64) MC {rate / 1e9:.1f} Gwords/s: LDS busy {busy:.3f}", flush=True)
+    assert busy <= 1.03
+
+
 def test_streaming_probes(ctx):
     # 16 passes of 16-byte stores / loads over 256 MiB, bytes per second
     for p in (6, 7):
@@ -61,6 +74,6 @@ def test_streaming_probes(ctx):
 
 def test_invalid_probe(ctx):
     with pytest.raises(cra.CRError):
-        ctx.probe_ceiling(8)
+        ctx.probe_ceiling(9)
     with pytest.raises(cra.CRError):
         ctx.probe_ceiling(0, 0)
