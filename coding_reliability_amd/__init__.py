@@ -274,6 +274,26 @@ class Context:
         check(lib().cr_an_mc(self._h, A, n, m, seed, _u64p(out), nbins, C.byref(ms)))
         return [int(x) for x in out], ms.value
 
+    def an_mc_ranges(self, A: int, n: int, bounds, seed: int, nbins: int = 65) -> tuple[list[list[int]], float]:
+        """Histograms of the sample ranges [bounds[i], bounds[i+1]) of an_mc
+        (same A, n, seed) in one launch (cr_an_mc_ranges)."""
+        b = np.ascontiguousarray(bounds, dtype=np.uint64)
+        count = len(b) - 1
+        out = np.zeros((max(count, 0), nbins), dtype=np.uint64)
+        ms = C.c_double(0)
+        check(lib().cr_an_mc_ranges(self._h, A, n, _u64p(b), count, seed, _u64p(out), nbins, C.byref(ms)))
+        return [[int(x) for x in row] for row in out], ms.value
+
+    def an_mc_prefixes(self, A: int, n: int, ms_list, seed: int, nbins: int = 65) -> tuple[list[list[int]], float]:
+        """an_mc histograms for each sample count in ms_list (non-decreasing),
+        every sample drawn once: the cumulative sums of an_mc_ranges."""
+        rows, ms = self.an_mc_ranges(A, n, [0, *ms_list], seed, nbins)
+        acc, out = [0] * nbins, []
+        for r in rows:
+            acc = [a + x for a, x in zip(acc, r)]
+            out.append(list(acc))
+        return out, ms
+
     def an_grid_ref(self, gdim: int, n: int, m1: int, m2: int, A: int, nbins: int = 65) -> tuple[list[int], float]:
         """Kernel-level histogram of the reference grid approximation
         (run_ancoding_grid) and the kernel time in ms."""

@@ -126,6 +126,8 @@ SIGNATURES = {
                                    C.POINTER(C.c_double)]),
     "cr_an_mc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint64, u64p, C.c_uint32,
                            C.POINTER(C.c_double)]),
+    "cr_an_mc_ranges": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint32, u64p, C.c_uint32, C.c_uint64, u64p,
+                                  C.c_uint32, C.POINTER(C.c_double)]),
 }
 
 

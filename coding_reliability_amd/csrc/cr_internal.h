@@ -206,8 +206,19 @@ uint32_t hamming_waves_per_cu(uint32_t lds_rows);
 cr_status launch_hamming(const HamJob *d_jobs, uint32_t njobs, uint64_t total_items, bool w64,
                          unsigned long long *counter, bool allow_static, uint32_t lds_rows, int cus,
                          hipStream_t stream, hipEvent_t ev0, hipEvent_t ev1);
-cr_status launch_an_mc(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, unsigned long long *counter,
-                       unsigned long long *out, uint32_t nbins, int cus, hipStream_t stream, hipEvent_t ev0,
+// A sample range of an AN Monte Carlo launch and its histogram: the pairs
+// (word i, sample j) for every word i and j0 <= j < j1 (j0 < j1).
+struct McSegment {
+    uint64_t j0, j1;
+    uint64_t item_begin;      // first work item of the range (set by launch_an_mc)
+    uint64_t chunks;          // sample chunks of the range (set by launch_an_mc)
+    unsigned long long *out;  // device histogram, nbins entries (zeroed by the caller)
+};
+constexpr uint32_t kMcMaxRanges = 64;  // sample ranges per AN Monte Carlo launch
+// segs (host, nsegs entries, kept alive until the stream is synchronized) is
+// completed and copied to d_segs; counter is the work-queue head (zeroed here).
+cr_status launch_an_mc(uint64_t A, uint32_t n, uint64_t seed, McSegment *segs, uint32_t nsegs, McSegment *d_segs,
+                       unsigned long long *counter, uint32_t nbins, int cus, hipStream_t stream, hipEvent_t ev0,
                        hipEvent_t ev1);
 
 // Zero-fills n 64-bit words on stream (a kernel, so it may be captured in graphs).

@@ -354,12 +354,16 @@ __global__ __launch_bounds__(64) void ham_kernel(const HamJob *__restrict__ jobs
 constexpr int RM = 4;
 constexpr uint32_t CJ = 256;  // samples per item (multiple of 4)
 
-// One AN Monte Carlo item.  W1: every codeword fits 32 bits (32-bit multiply,
-// xor and popcount).  S64: 64-bit samples (n > 32).  CHECK: rows past 2^n or
-// a partial sample chunk go to the dummy row.
+// One AN Monte Carlo item: samples [jlo, j1) of the chunk that starts at j0
+// (a multiple of CJ, so of the words per Philox block).  W1: every codeword
+// fits 32 bits (32-bit multiply, xor and popcount).  S64: 64-bit samples
+// (n > 32).  CHECK: rows past 2^n and samples outside [jlo, j1) (a partial
+// chunk) go to the dummy row; without CHECK, jlo = j0 and j1 - j0 is a
+// multiple of the words per Philox block.
 template <bool W1, bool S64, bool CHECK>
 __device__ __forceinline__ void an_mc_item(uint32_t *lane_col, uint64_t A, uint64_t rb, uint64_t nwords, uint64_t j0,
-                                           uint64_t j1, uint64_t smask, uint32_t k0, uint32_t k1, uint32_t dummy) {
+                                           uint64_t jlo, uint64_t j1, uint64_t smask, uint32_t k0, uint32_t k1,
+                                           uint32_t dummy) {
     const uint32_t lane = threadIdx.x;
     constexpr uint32_t per = S64 ? 2 : 4;
 #pragma unroll
@@ -368,7 +372,7 @@ __device__ __forceinline__ void an_mc_item(uint32_t *lane_col, uint64_t A, uint6
         const bool ok = !CHECK || i < nwords;
         const uint64_t w = A * i;
         const uint32_t tag = S64 ? kMcTag ^ (uint32_t)(i >> 32) : kMcTag;
-        for (uint64_t j = j0; j < j1; j += per) {
+        for (uint64_t j = CHECK ? jlo / per * per : j0; j < j1; j += per) {
             const uint64_t blk = j / per;
             uint32_t c[4] = {(uint32_t)blk, (uint32_t)(blk >> 32), (uint32_t)i, tag};
             philox4x32_10(c, k0, k1);
@@ -377,7 +381,7 @@ __device__ __forceinline__ void an_mc_item(uint32_t *lane_col, uint64_t A, uint6
                 for (uint32_t u = 0; u < 2; ++u) {
                     const uint64_t x = ((uint64_t)c[2 * u] | ((uint64_t)c[2 * u + 1] << 32)) & smask;
                     const uint32_t d = (uint32_t)__popcll(w ^ (A * x));
-                    lds_inc(lane_col + ((!CHECK || (ok && j + u < j1)) ? d : dummy) * kStride);
+                    lds_inc(lane_col + ((!CHECK || (ok && j + u >= jlo && j + u < j1)) ? d : dummy) * kStride);
                 }
             } else {
 #pragma unroll
@@ -385,21 +389,25 @@ __device__ __forceinline__ void an_mc_item(uint32_t *lane_col, uint64_t A, uint6
                     const uint32_t x = c[u] & (uint32_t)smask;
                     const uint32_t d = W1 ? (uint32_t)__popc((uint32_t)w ^ ((uint32_t)A * x))
                                           : (uint32_t)__popcll(w ^ (A * (uint64_t)x));
-                    lds_inc(lane_col + ((!CHECK || (ok && j + u < j1)) ? d : dummy) * kStride);
+                    lds_inc(lane_col + ((!CHECK || (ok && j + u >= jlo && j + u < j1)) ? d : dummy) * kStride);
                 }
             }
         }
     }
 }
 
-// Item t = (row group t / SC of G row blocks, sample chunk t % SC): G is
-// chosen on the host so that an item holds about 2^18 pairs whatever m is
-// (one work-queue step per item).
-__global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint32_t G,
-                                                   uint64_t SC, uint64_t total_items,
-                                                   unsigned long long *__restrict__ counter,
-                                                   unsigned long long *__restrict__ out, uint32_t nbins,
-                                                   uint32_t lds_rows) {
+// Item t of segment s (McSegment: samples [j0, j1) with a histogram of its
+// own) = (row group of G row blocks, sample chunk): local index t -
+// item_begin = rg * chunks + c, chunk c covering [a + c*CJ, a + (c+1)*CJ) of
+// [j0, j1), a = j0 rounded down to CJ.  G is chosen on the host so that an
+// item holds about 2^18 pairs (one work-queue step per item).  Segments are
+// laid out one after the other, so a wave's items stay in one segment for
+// long runs; its LDS rows go to the segment's histogram when the segment
+// changes.
+__global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint64_t seed, uint32_t G,
+                                                   const McSegment *__restrict__ segs, uint32_t nsegs,
+                                                   uint64_t total_items, unsigned long long *__restrict__ counter,
+                                                   uint32_t nbins, uint32_t lds_rows) {
     extern __shared__ uint32_t lds[];
     const uint32_t lane = threadIdx.x;
     lds_clear(lds, lds_rows);
@@ -413,37 +421,45 @@ __global__ __launch_bounds__(64) void an_mc_kernel(uint64_t A, uint32_t n, uint6
     const bool w1 = ((unsigned __int128)A * (nwords - 1)) >> 32 == 0;
     const bool s64 = n > 32;
     uint64_t pending = 0;  // increments per lane counter since the last flush (bound)
-    bool dirty = false;
+    uint32_t cur = 0xFFFFFFFFu;  // segment whose counts the LDS rows hold
     for (uint64_t t = first_item(); t < total_items; t = following_item(counter, t)) {
-        const uint64_t rg = uniform64(t / SC), sc = uniform64(t - (t / SC) * SC);
-        const uint64_t j0 = sc * CJ;
-        const uint64_t j1 = j0 + CJ < m ? j0 + CJ : m;
+        uint32_t sg = cur == 0xFFFFFFFFu ? 0 : cur;
+        while (sg + 1 < nsegs && t >= segs[sg + 1].item_begin) ++sg;
+        while (t < segs[sg].item_begin) --sg;
+        sg = uniform32(sg);
+        const McSegment &seg = segs[sg];
+        const uint64_t local = t - seg.item_begin;
+        const uint64_t rg = uniform64(local / seg.chunks), c = uniform64(local - rg * seg.chunks);
+        const uint64_t j0 = (seg.j0 / CJ + c) * CJ;
+        const uint64_t jlo = j0 > seg.j0 ? j0 : seg.j0;
+        const uint64_t j1 = j0 + CJ < seg.j1 ? j0 + CJ : seg.j1;
         const uint64_t add = (uint64_t)G * RM * (j1 - j0);
-        if (pending + add >= 0xFFFFFFFFull) {
-            flush_rows(lds, 0, nbins, out);
+        if (sg != cur || pending + add >= 0xFFFFFFFFull) {
+            if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, nbins, segs[cur].out);
+            cur = sg;
             pending = 0;
         }
         pending += add;
-        dirty = true;
         for (uint32_t g = 0; g < G; ++g) {
             const uint64_t rb = rg * G + g;
             if (rb >= RBlocks) break;
-            const bool full = (rb + 1) * (64ull * RM) <= nwords && j1 - j0 == CJ;
+            // whole rows and whole Philox blocks: no masking
+            const bool full = (rb + 1) * (64ull * RM) <= nwords && jlo == j0 && (j1 - j0) % (s64 ? 2 : 4) == 0;
             if (s64) {
                 if (full)
-                    an_mc_item<false, true, false>(lane_col, A, rb, nwords, j0, j1, smask, k0, k1, dummy);
+                    an_mc_item<false, true, false>(lane_col, A, rb, nwords, j0, jlo, j1, smask, k0, k1, dummy);
                 else
-                    an_mc_item<false, true, true>(lane_col, A, rb, nwords, j0, j1, smask, k0, k1, dummy);
+                    an_mc_item<false, true, true>(lane_col, A, rb, nwords, j0, jlo, j1, smask, k0, k1, dummy);
             } else if (w1 && full) {
-                an_mc_item<true, false, false>(lane_col, A, rb, nwords, j0, j1, smask, k0, k1, dummy);
+                an_mc_item<true, false, false>(lane_col, A, rb, nwords, j0, jlo, j1, smask, k0, k1, dummy);
             } else if (full) {
-                an_mc_item<false, false, false>(lane_col, A, rb, nwords, j0, j1, smask, k0, k1, dummy);
+                an_mc_item<false, false, false>(lane_col, A, rb, nwords, j0, jlo, j1, smask, k0, k1, dummy);
             } else {
-                an_mc_item<false, false, true>(lane_col, A, rb, nwords, j0, j1, smask, k0, k1, dummy);
+                an_mc_item<false, false, true>(lane_col, A, rb, nwords, j0, jlo, j1, smask, k0, k1, dummy);
             }
         }
     }
-    if (dirty) flush_rows(lds, 0, nbins, out);
+    if (cur != 0xFFFFFFFFu) flush_rows(lds, 0, nbins, segs[cur].out);
 }
 
 // Dynamic LDS of a Hamming wave: the histogram rows (the syndrome tables are
@@ -502,35 +518,49 @@ uint32_t hamming_waves_per_cu(uint32_t lds_rows) {
     return (uint32_t)std::max(std::max(a, b), 1);
 }
 
-cr_status launch_an_mc(uint64_t A, uint32_t n, uint64_t m, uint64_t seed, unsigned long long *counter,
-                       unsigned long long *out, uint32_t nbins, int cus, hipStream_t stream, hipEvent_t ev0,
+cr_status launch_an_mc(uint64_t A, uint32_t n, uint64_t seed, McSegment *segs, uint32_t nsegs, McSegment *d_segs,
+                       unsigned long long *counter, uint32_t nbins, int cus, hipStream_t stream, hipEvent_t ev0,
                        hipEvent_t ev1) {
     const uint32_t lds_rows = nbins + 1;
     if (lds_rows > 66) return fail(CR_ERR_INVALID_ARGUMENT, "bad histogram rows %u", lds_rows);
+    if (nsegs == 0) return fail(CR_ERR_INVALID_ARGUMENT, "no sample range");
     const void *k = reinterpret_cast<const void *>(&an_mc_kernel);
     const size_t lds = (size_t)lds_rows * kStride * sizeof(uint32_t);
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 64, lds) != hipSuccess || per_cu <= 0)
         return fail(CR_ERR_DEVICE, "occupancy query failed for the AN Monte Carlo kernel");
     const uint64_t RBlocks = ((1ull << n) + 64ull * RM - 1) / (64ull * RM);
-    const uint64_t SC = (m + CJ - 1) / CJ;
+    uint64_t total_chunks = 0, longest = 0;
+    for (uint32_t s = 0; s < nsegs; ++s) {
+        if (segs[s].j1 <= segs[s].j0) return fail(CR_ERR_INVALID_ARGUMENT, "empty sample range");
+        segs[s].chunks = (segs[s].j1 - segs[s].j0 / CJ * CJ + CJ - 1) / CJ;
+        total_chunks += segs[s].chunks;
+        longest = std::max(longest, segs[s].j1 - segs[s].j0);
+    }
     const uint64_t resident = (uint64_t)per_cu * cus;
     // row blocks per item: about 2^18 pairs per item (a power of two up to
     // 4096), keeping at least 16 items per resident wave
-    const uint64_t chunk = m < CJ ? m : CJ;
+    const uint64_t chunk = longest < CJ ? longest : CJ;
     uint32_t G = 1;
     while (G < 4096 && (uint64_t)G * 64 * RM * chunk < (1ull << 18) &&
-           ((RBlocks + 2 * G - 1) / (2 * G)) * SC >= 16 * resident)
+           ((RBlocks + 2 * G - 1) / (2 * G)) * total_chunks >= 16 * resident)
         G *= 2;
-    const uint64_t items = ((RBlocks + G - 1) / G) * SC;
+    const uint64_t RG = (RBlocks + G - 1) / G;
+    uint64_t items = 0;
+    for (uint32_t s = 0; s < nsegs; ++s) {
+        segs[s].item_begin = items;
+        items += RG * segs[s].chunks;
+    }
     uint64_t waves = resident;
     if (waves > items) waves = items;
+    CR_HIP(hipMemcpyAsync(d_segs, segs, nsegs * sizeof(McSegment), hipMemcpyHostToDevice, stream));
     CR_TRY(fill_zero_u64(counter, 1, stream));
     if (ev0) CR_HIP(hipEventRecord(ev0, stream));
-    uint64_t a_A = A, a_m = m, a_seed = seed, a_SC = SC, a_items = items;
-    uint32_t a_n = n, a_G = G, a_nbins = nbins, a_rows = lds_rows;
-    unsigned long long *a_counter = counter, *a_out = out;
-    void *args[] = {&a_A, &a_n, &a_m, &a_seed, &a_G, &a_SC, &a_items, &a_counter, &a_out, &a_nbins, &a_rows};
+    uint64_t a_A = A, a_seed = seed, a_items = items;
+    uint32_t a_n = n, a_G = G, a_nsegs = nsegs, a_nbins = nbins, a_rows = lds_rows;
+    const McSegment *a_segs = d_segs;
+    unsigned long long *a_counter = counter;
+    void *args[] = {&a_A, &a_n, &a_seed, &a_G, &a_segs, &a_nsegs, &a_items, &a_counter, &a_nbins, &a_rows};
     CR_HIP(hipLaunchKernel(k, dim3((uint32_t)waves), dim3(64), args, lds, stream));
     if (ev1) CR_HIP(hipEventRecord(ev1, stream));
     return CR_OK;

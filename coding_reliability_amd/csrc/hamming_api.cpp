@@ -161,38 +161,66 @@ extern "C" cr_status cr_hamming_report(uint32_t k, const uint64_t *weights, int 
     return CR_OK;
 }
 
-extern "C" cr_status cr_an_mc(cr_context *ctx, uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint64_t *hist,
-                              uint32_t nbins, double *kernel_ms) {
-    clear_error();
-    if (!ctx || !hist) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+namespace {
+
+// Histograms of the sample ranges [bounds[i], bounds[i+1]), i < count, in one
+// launch: hists[i * nbins + b].  Empty ranges stay zero.
+cr_status an_mc_ranges(cr_context *ctx, uint64_t A, uint32_t n, const uint64_t *bounds, uint32_t count,
+                       uint64_t seed, uint64_t *hists, uint32_t nbins, double *kernel_ms) {
+    if (!ctx || !hists || !bounds) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
     if (A == 0) return fail(CR_ERR_INVALID_ARGUMENT, "A must be >= 1");
     if (n < 1 || n > 63) return fail(CR_ERR_UNSUPPORTED, "AN Monte Carlo supports 1 <= n <= 63 (got %u)", n);
     if ((unsigned __int128)A * ((1ull << n) - 1) >> 64)
         return fail(CR_ERR_UNSUPPORTED, "codewords A*(2^n-1) exceed 64 bits");
+    if (count < 1 || count > kMcMaxRanges)
+        return fail(CR_ERR_INVALID_ARGUMENT, "%u sample ranges outside [1, %u]", count, kMcMaxRanges);
+    for (uint32_t i = 0; i < count; ++i)
+        if (bounds[i + 1] < bounds[i]) return fail(CR_ERR_INVALID_ARGUMENT, "sample bounds decrease at %u", i + 1);
     uint32_t bits = 0;
     for (unsigned __int128 x = (unsigned __int128)A * ((1ull << n) - 1); x; x >>= 1) ++bits;
     const uint32_t need = bits + 1;
     if (nbins < need) return fail(CR_ERR_INVALID_ARGUMENT, "nbins=%u but distances reach %u", nbins, need - 1);
     CR_HIP(hipSetDevice(ctx->device));
-    memset(hist, 0, nbins * sizeof(uint64_t));
-    if (m == 0) {
-        if (kernel_ms) *kernel_ms = 0;
-        return CR_OK;
-    }
-    CR_TRY(ctx->hist.reserve(need * sizeof(uint64_t)));
+    memset(hists, 0, (size_t)count * nbins * sizeof(uint64_t));
+    if (kernel_ms) *kernel_ms = 0;
+    const uint64_t entries = (uint64_t)count * need;
+    CR_TRY(ctx->hist.reserve(entries * sizeof(uint64_t)));
     CR_TRY(ctx->counters.reserve(sizeof(uint64_t)));
+    CR_TRY(ctx->jobs.reserve(count * sizeof(McSegment)));
     unsigned long long *out = ctx->hist.as<unsigned long long>();
-    CR_TRY(fill_zero_u64(out, need, ctx->stream));
-    CR_TRY(launch_an_mc(A, n, m, seed, ctx->counters.as<unsigned long long>(), out, need, ctx->cus, ctx->stream,
+    std::vector<McSegment> segs;
+    for (uint32_t i = 0; i < count; ++i)
+        if (bounds[i + 1] > bounds[i]) segs.push_back({bounds[i], bounds[i + 1], 0, 0, out + (uint64_t)i * need});
+    if (segs.empty()) return CR_OK;
+    CR_TRY(fill_zero_u64(out, entries, ctx->stream));
+    CR_TRY(launch_an_mc(A, n, seed, segs.data(), (uint32_t)segs.size(), ctx->jobs.as<McSegment>(),
+                        ctx->counters.as<unsigned long long>(), need, ctx->cus, ctx->stream,
                         kernel_ms ? ctx->ev0 : nullptr, kernel_ms ? ctx->ev1 : nullptr));
-    CR_TRY(ctx->host.reserve(need * sizeof(uint64_t)));
-    CR_HIP(hipMemcpyAsync(ctx->host.ptr, out, need * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    CR_TRY(ctx->host.reserve(entries * sizeof(uint64_t)));
+    CR_HIP(hipMemcpyAsync(ctx->host.ptr, out, entries * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
     CR_HIP(hipStreamSynchronize(ctx->stream));
-    memcpy(hist, ctx->host.ptr, need * sizeof(uint64_t));
+    const uint64_t *h = ctx->host.as<uint64_t>();
+    for (uint32_t i = 0; i < count; ++i) memcpy(hists + (uint64_t)i * nbins, h + (uint64_t)i * need, need * sizeof(uint64_t));
     if (kernel_ms) {
         float ms = 0;
         CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         *kernel_ms = ms;
     }
     return CR_OK;
+}
+
+}  // namespace
+
+extern "C" cr_status cr_an_mc(cr_context *ctx, uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint64_t *hist,
+                              uint32_t nbins, double *kernel_ms) {
+    clear_error();
+    const uint64_t bounds[2] = {0, m};
+    return an_mc_ranges(ctx, A, n, bounds, 1, seed, hist, nbins, kernel_ms);
+}
+
+extern "C" cr_status cr_an_mc_ranges(cr_context *ctx, uint64_t A, uint32_t n, const uint64_t *bounds,
+                                     uint32_t count, uint64_t seed, uint64_t *hists, uint32_t nbins,
+                                     double *kernel_ms) {
+    clear_error();
+    return an_mc_ranges(ctx, A, n, bounds, count, seed, hists, nbins, kernel_ms);
 }

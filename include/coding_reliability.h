@@ -242,6 +242,19 @@ cr_status cr_hamming_sdc_table(cr_context *ctx, uint32_t k, int method, uint64_t
 cr_status cr_an_mc(cr_context *ctx, uint64_t A, uint32_t n, uint64_t m, uint64_t seed, uint64_t *hist, uint32_t nbins,
                    double *kernel_ms);
 
+/* AN Monte Carlo histograms of consecutive sample ranges in one launch:
+ * bounds[0] <= bounds[1] <= ... <= bounds[count] (1 <= count <= 64), and
+ * hists[i * nbins + b] counts the pairs (i', j) of cr_an_mc (same A, n, seed)
+ * at distance b with bounds[i] <= j < bounds[i+1].  Sample j is the same draw
+ * in every call with this seed, so cr_an_mc with m samples equals the sum of
+ * the ranges below m, and each sample is drawn once however many ranges there
+ * are.  With bounds {0, m, 3m, 9m, ...} the cumulative sums are the histograms
+ * of ancoding_mc_search's successive run_ancoding_mc calls
+ * (distance_distribution/src/main.cpp:179-202), which otherwise redraw every
+ * earlier sample at each step. */
+cr_status cr_an_mc_ranges(cr_context *ctx, uint64_t A, uint32_t n, const uint64_t *bounds, uint32_t count,
+                          uint64_t seed, uint64_t *hists, uint32_t nbins, double *kernel_ms);
+
 /* --- batched exhaustive AN ---------------------------------------------------------
  * Many codes in one persistent launch.  Code c is (A_c, num_words_c); its raw
  * histogram (as cr_an_pairs) is written to raw[c*stride .. c*stride+stride).

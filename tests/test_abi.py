@@ -61,6 +61,8 @@ def test_null_arguments_are_refused_without_gpu():
     assert L.cr_last_error() == b"NULL argument"
     assert L.cr_hamming_report(8, null, 1, null) == 1
     assert L.cr_probe_ceiling(null, 0, 16, null, null) == 1
+    assert L.cr_an_mc(null, 61, 16, 10, 1, null, 65, null) == 1
+    assert L.cr_an_mc_ranges(null, 61, 16, null, 1, 1, null, 65, null) == 1
 
 
 def test_context_without_device_is_an_error():

@@ -158,6 +158,46 @@ def test_an_mc_64bit_binomial_closed_form(ctx, A, n, m):
     assert not any(h[n + 1:])
 
 
+@pytest.mark.parametrize("A,n,bounds", [(61, 10, [0, 7, 100, 333, 1024]), (58659, 12, [5, 5, 64, 257, 700]),
+                                        (3, 9, [0, 1, 2, 3, 4, 255, 256, 257])])
+def test_an_mc_ranges_bitexact_vs_oracle(ctx, A, n, bounds):
+    # each range [b_i, b_{i+1}) is the difference of two oracle prefixes;
+    # bounds off the 4-word Philox blocks and the 256-sample chunks, and an
+    # empty range, exercise the partial chunks
+    rows, _ = ctx.an_mc_ranges(A, n, bounds, seed=99)
+    pre = [O.an_mc(A, n, b, 99) for b in bounds]
+    for i, r in enumerate(rows):
+        want = [x - y for x, y in zip(pre[i + 1], pre[i])]
+        assert r == want[: len(r)], (i, bounds[i], bounds[i + 1])
+
+
+@pytest.mark.parametrize("A,n,ms", [(61, 16, [1000, 3000, 9000, 27000]), (58659, 20, [7, 21, 63, 189, 567]),
+                                    (61, 33, [1, 3, 9])])
+def test_an_mc_prefixes_equal_separate_calls(ctx, A, n, ms):
+    # the search's nested prefixes from one launch (every sample drawn once)
+    # equal separate an_mc calls with each sample count, bit for bit; n = 33
+    # draws 64-bit samples, two per Philox block
+    pre, t_one = ctx.an_mc_prefixes(A, n, ms, seed=16384)
+    t_sep = 0.0
+    for m, p in zip(ms, pre):
+        h, t = ctx.an_mc(A, n, m, seed=16384)
+        t_sep += t
+        assert p == h, m
+        assert sum(p) == (1 << n) * m
+    print(f"A={A} n={n} prefixes {ms}: one launch {t_one:.2f} ms, separate calls {t_sep:.2f} ms", flush=True)
+
+
+def test_an_mc_ranges_refusals(ctx):
+    with pytest.raises(cra.CRError):
+        ctx.an_mc_ranges(61, 10, [10, 5], seed=1)  # decreasing
+    with pytest.raises(cra.CRError):
+        ctx.an_mc_ranges(61, 10, [0], seed=1)  # no range
+    with pytest.raises(cra.CRError):
+        ctx.an_mc_ranges(61, 10, list(range(66)), seed=1)  # 65 ranges
+    rows, ms = ctx.an_mc_ranges(61, 10, [4, 4, 4], seed=1)  # all empty: nothing launched
+    assert rows == [[0] * 65, [0] * 65] and ms == 0
+
+
 def test_an_mc_4sigma_vs_exhaustive(ctx):
     A, n, m = 58659, 16, 4096
     h, _ = ctx.an_mc(A, n, m, seed=77)

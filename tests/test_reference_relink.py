@@ -340,3 +340,33 @@ def test_cli_an_mc_beyond_32_bits(tmp_path):
     counts = [float(r.split(",")[1]) for r in rows]
     assert counts[1] == 0  # no two multiples of 61 differ in one bit
     assert abs(sum(counts) / 4 ** 40 - 1) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(not os.path.exists(CLI), reason="relinked reference CLI not built")
+def test_cli_mc_search_steps_equal_separate_runs(tmp_path, solutions):
+    # ancoding_mc_search (main.cpp:179-202): run_ancoding_mc at m, 3m, 9m, ...
+    # until the max relative error (offset 0, an_coding_mc.cu's return value)
+    # is within the bound.  The drop-in draws each sample once across the
+    # steps; every step's error must equal the one a fresh process computes
+    # from its own run_ancoding_mc counts at that iteration count.
+    A, n, m0, bound = 61, 16, 1000, 0.05
+    run_cli(tmp_path, "-a", str(A), "-n", str(n), "-m", str(m0), "-s", str(bound))
+    rows = [ln.split(",") for ln in open(tmp_path / "ancoding_mc_search.csv").read().split("\n") if ln.strip()]
+    assert len(rows) >= 2
+    sol = solutions["solution_an16_A61"]
+    for step, r in enumerate(rows):
+        assert int(r[0]) == step and int(r[1]) == n and int(r[3]) == m0 * 3 ** step
+        iters = int(r[3])
+        sub = tmp_path / f"step{step}"
+        sub.mkdir()
+        run_cli(sub, "-a", str(A), "-n", str(n), "-m", str(iters), "-f")
+        # run_ancoding_mc caps the iterations at 2^n (an_coding_mc.cu) and
+        # names the file by the capped count
+        counts = read_counts(sub / f"ancoding_mc_gpu_m{min(iters, 1 << n)}_A{A}_n{n}_h6.csv")
+        err = 0.0
+        for k in range(min(n + 6, len(sol))):
+            e = abs(counts[k] / sol[k] - 1.0) if sol[k] > 0 else (1.0 if counts[k] > 0 else 0.0)
+            err = max(err, e)
+        assert abs(float(r[4]) - err) <= 1e-6, (step, r[4], err)
+        assert (err <= bound) == (step == len(rows) - 1)
