@@ -169,22 +169,32 @@ extern "C" cr_status cr_an_pairs_spectral(cr_context *ctx, uint64_t A, uint32_t 
         return fail(CR_ERR_UNSUPPORTED, "spectral path: A=%llu at k=%u (odd part below 2^32, codewords of at most "
                     "min(2k, 48) bits)", (unsigned long long)A, k);
     CR_HIP(hipSetDevice(ctx->device));
-    CR_TRY(ctx->table_a.reserve(p.buffer_bytes));
-    CR_TRY(ctx->hist.reserve(p.slot_entries * sizeof(uint64_t)));
-    CR_TRY(ctx->host.reserve(p.slot_entries * sizeof(uint64_t)));
-    unsigned long long *slots = ctx->hist.as<unsigned long long>();
-    CR_TRY(fill_zero_u64(slots, p.slot_entries, ctx->stream));
-    if (kernel_ms) CR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    CR_TRY(spectral_run(p, ctx->table_a.ptr, slots, 0, p.batches, ctx->stream));
-    if (kernel_ms) CR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-    CR_HIP(hipMemcpyAsync(ctx->host.ptr, slots, p.slot_entries * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    CR_HIP(hipStreamSynchronize(ctx->stream));
-    CR_TRY(spectral_finish(p, ctx->host.as<uint64_t>(), raw, nbins));
-    if (kernel_ms) {
-        float ms = 0;
-        CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-        *kernel_ms = ms;
+    const cr_status st = [&]() -> cr_status {
+        CR_TRY(ctx->table_a.reserve(p.buffer_bytes));
+        CR_TRY(ctx->hist.reserve(p.slot_entries * sizeof(uint64_t)));
+        CR_TRY(ctx->host.reserve(p.slot_entries * sizeof(uint64_t)));
+        unsigned long long *slots = ctx->hist.as<unsigned long long>();
+        CR_TRY(fill_zero_u64(slots, p.slot_entries, ctx->stream));
+        if (kernel_ms) CR_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        CR_TRY(spectral_run(p, ctx->table_a.ptr, slots, 0, p.batches, ctx->stream));
+        if (kernel_ms) CR_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        CR_HIP(hipMemcpyAsync(ctx->host.ptr, slots, p.slot_entries * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        CR_HIP(hipStreamSynchronize(ctx->stream));
+        CR_TRY(spectral_finish(p, ctx->host.as<uint64_t>(), raw, nbins));
+        if (kernel_ms) {
+            float ms = 0;
+            CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            *kernel_ms = ms;
+        }
+        return CR_OK;
+    }();
+    // transform buffers above 1 GiB (16 GiB at k = 32) go back after the call:
+    // a long-lived context (the reference entry points keep one per device)
+    // would otherwise hold them until it is destroyed
+    if (p.buffer_bytes > (1ull << 30)) {
+        (void)hipStreamSynchronize(ctx->stream);
+        ctx->table_a.release();
     }
-    return CR_OK;
+    return st;
 }

@@ -49,6 +49,16 @@ def test_narrow_last_pass_matches_pair_kernel(ctx, A, k):
     assert raw_s == raw_p
 
 
+def test_k25_multi_transform_matches_pair_kernel(ctx):
+    # A = 61 at k = 25: 31-bit codewords, 64 transforms of 2^25 through the
+    # middle pass and the narrow last pass (per-transform weight base and high
+    # bits), bit for bit against the pair kernel's 2^49 pairs (about a minute)
+    raw_s, ms_s = ctx.an_pairs_spectral(61, 25)
+    raw_p, ms_p = ctx.an_pairs(61, 1 << 25)
+    print(f"k=25 A=61: spectral {ms_s:.1f} ms, pair kernel {ms_p:.1f} ms", flush=True)
+    assert raw_s == raw_p
+
+
 @pytest.mark.parametrize("k", [25, 26, 27, 28, 29, 30, 31])
 def test_closed_form_k25_to_31(ctx, k):
     # A = 1: D(d) = 2^k C(k, d) (the middle pass and the narrow last pass)
@@ -122,11 +132,15 @@ def test_k32_closed_form(ctx, A):
     # all 2^32 words (odd part 1): D(d) = 2^32 C(32, d), so raw[d] = 2^31 C(32, d);
     # the transform reaches F = 2^32 at S = 0, the 64-bit end of the last pass
     import math
+    free_before, _ = ctx.device_memory()
     raw, ms = ctx.an_pairs_spectral(A, 32)
     print(f"k=32 A={A}: {ms:.1f} ms", flush=True)
     assert raw[0] == 0
     assert raw[1:33] == [(1 << 31) * math.comb(32, d) for d in range(1, 33)]
     assert all(x == 0 for x in raw[33:])
+    # the 16 GiB transform buffer is not kept by the context
+    free_after, _ = ctx.device_memory()
+    assert free_after >= free_before - (64 << 20), (free_before, free_after)
 
 
 @pytest.mark.parametrize("A", [61, 3])
