@@ -101,7 +101,7 @@ def cpu_baseline(A: int, k: int, target_s: float) -> dict:
     }
 
 
-def run_batched(ctx, autotune: bool = True) -> dict:
+def run_batched(ctx, ceil: dict | None = None, autotune: bool = True) -> dict:
     """4096 AN codes (odd A uniform in [3, 2^16), k uniform in 8..18, seed
     MIXED_CODES_SEED) -- one code per launch (cr_an_pairs per code: zero, kernel,
     copy-back, sync) against one batched launch and its captured hipGraph;
@@ -145,6 +145,21 @@ def run_batched(ctx, autotune: bool = True) -> dict:
         "speedup_graph_vs_per_launch": round(t_per / t_graph, 3),
         "tuning": tune, "bit_exact": True,
     }
+    # the bound batching can reach on this mix: its codes with k >= 17 hold
+    # most of the pairs and each fills the device alone, so a batched launch
+    # can at best run every pair at the LDS-increment ceiling; what batching
+    # removes is the per-code launch and synchronisation time
+    big = WL.pairs_of([(A, N) for A, N in codes if N >= 1 << 17])
+    if ceil:
+        t_floor = pairs / ceil["lds"]
+        out["codes4096"].update({
+            "share_of_pairs_k17_18": round(big / pairs, 4),
+            "batched_floor_s": round(t_floor, 4),
+            "speedup_bound": round(t_per / t_floor, 3),
+            "bound_note": "batched_floor_s = pairs / the run's LDS-increment ceiling (one increment per pair); the "
+                          "per-launch run is already within per_launch_s - batched_floor_s of it, so no batching "
+                          "of this mix can exceed speedup_bound",
+        })
     # the reference's published per-launch sweep: super-A k=8, 16384 codes
     sweep = WL.supera_codes(8, 16)
     spairs = WL.pairs_of(sweep)
@@ -682,7 +697,7 @@ def main() -> int:
     batched = None
     if not args.no_batched:
         log("batched vs per-launch ...")
-        batched = run_batched(ctx)
+        batched = run_batched(ctx, ceil)
 
     kernels = None
     if not args.no_kernels:
