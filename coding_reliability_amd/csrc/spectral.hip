@@ -364,6 +364,12 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
 // pass transforms index bits 0-13 (values within +-2^14 fit int16) of two
 // chunks at once, the last pass the remaining 10 bits.
 //
+// The passes transform the parities q(x) = parity(s & h(x)) in {0, 1} rather
+// than the signs (-1)^q = 1 - 2q: the transform of 1 - 2q is 2^k delta(S) -
+// 2 Q(S), Q the transform of q, so F^2 / 4 = Q^2 except at S = 0, where it is
+// (Q(0) - 2^(k-1))^2.  The last pass sums F^2 / 4 and the host multiplies the
+// weight sums by 4.  After 14 stages |Q| <= 2^14, within int16.
+//
 // spec_gen14p: workgroup (G, y) of 4 waves takes the 2^14 elements x_lo =
 // G * 2^14 + e of chunks sa = s0 + 2y (even) and sa + 1, each dword of its
 // working set holding the two chunks' values as int16 halves (packed
@@ -371,13 +377,14 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
 // Since sa + 1 differs from sa in bit 0 only, parity((sa + 1) & h) =
 // parity(sa & h) ^ (h & 1).  Thread (wave w, lane l) generates e = w * 4096
 // + l * 64 + j (j < 64): bits 0-5 in registers; bits 6-11 after a transpose
-// within the wave through LDS (lane l, register j at l * 64 + (j ^ l):
-// conflict free both ways); bits 12-13 after an exchange across the waves in
-// two rounds of 32 registers.  Each chunk's result is written as int16 in the
+// within the wave through LDS (rows of 65 dwords: conflict free both ways);
+// bits 12-13 after an exchange across the waves in two rounds of 32
+// registers.  Each chunk's result is written as int16 in the
 // layout Y[c][S_lo >> 6][x_hi][S_lo & 63] (x_hi < 2^(k-14)), so that a tile of
 // the last pass -- all x_hi of 32 consecutive S_lo -- is one contiguous run of
 // 64-byte rows.
 typedef short v2s __attribute__((ext_vector_type(2)));
+typedef int v4i __attribute__((ext_vector_type(4)));  // a buffer descriptor in four SGPRs
 __device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(v2s, a) + __builtin_bit_cast(v2s, b));
 }
@@ -401,7 +408,7 @@ __device__ __forceinline__ void wht_pk(uint32_t (&v)[64]) {
 constexpr uint32_t kGen14Threads = 256;
 __global__ __launch_bounds__(kGen14Threads, 2) void spec_gen14p(int16_t *__restrict__ Y, uint32_t k, uint32_t A,
                                                                  uint32_t Ainv, uint32_t tc, uint32_t s0) {
-    __shared__ uint32_t L[4 * 64 * 64];  // 64 KB: a 64 x 64 transpose per wave, then the exchange
+    __shared__ uint32_t L[4 * 64 * 65];  // 65 KB: a 64 x 64 transpose per wave (rows of 65), then the exchange
     const uint32_t t = threadIdx.x, l = t & 63, w = t >> 6;
     const uint32_t mask = (1u << k) - 1;
     const uint32_t G = blockIdx.x;
@@ -413,17 +420,21 @@ __global__ __launch_bounds__(kGen14Threads, 2) void spec_gen14p(int16_t *__restr
 #pragma unroll
     for (int j = 0; j < 64; ++j) {
         const uint32_t pa = __popc(h & sa);
-        const uint32_t q = (pa | ((pa ^ h) << 16)) & 0x00010001u;  // parities of chunks sa (low) and sa+1 (high)
-        v[j] = 0x00010001u ^ (q * 0xFFFEu);                       // per half: 0 -> +1 (0x0001), 1 -> -1 (0xFFFF)
+        v[j] = (pa | ((pa ^ h) << 16)) & 0x00010001u;  // parities of chunks sa (low) and sa+1 (high)
         h = next_high(h, tc, A);
     }
     wht_pk<1, 64>(v);  // bits 0-5; |v| <= 2^6
-    uint32_t *R = L + w * 4096;
+    // transpose within the wave through rows of 65 dwords: lane l's register
+    // j at R[65 l + j], read back by lane l' as register r from R[65 r + l'];
+    // the register index is an immediate offset and the banks, (l + j) and
+    // (r + l') mod 32, differ across the lanes of each access.  A wave's LDS
+    // accesses execute in order, so no barrier separates the two.
+    uint32_t *R = L + w * (64 * 65);
 #pragma unroll
-    for (int j = 0; j < 64; ++j) R[l * 64 + ((uint32_t)j ^ l)] = v[j];
-    __syncthreads();
+    for (int j = 0; j < 64; ++j) R[l * 65 + j] = v[j];
+    __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int r = 0; r < 64; ++r) v[r] = R[(uint32_t)r * 64 + (l ^ (uint32_t)r)];
+    for (int r = 0; r < 64; ++r) v[r] = R[(uint32_t)r * 65 + l];
     wht_pk<1, 64>(v);  // bits 6-11 (v[r]: bits 6-11 = r, bits 0-5 = l); |v| <= 2^12
     // bits 12-13: after round hr, v[32 hr + 8 q + rr] holds register
     // 32 hr + 8 w + rr of wave q
@@ -444,8 +455,15 @@ __global__ __launch_bounds__(kGen14Threads, 2) void spec_gen14p(int16_t *__restr
     const uint32_t X = 1u << (k - 14);
     int16_t *Ya = Y + ((uint64_t)(2 * blockIdx.y) << k);
     const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(Ya, 0, (int)(2u << k), 0x00020000);
-    const __amdgpu_buffer_rsrc_t rb =
-        __builtin_amdgcn_make_buffer_rsrc(Ya + ((uint64_t)1 << k), 0, (int)(2u << k), 0x00020000);
+    // chunk sa + 1 takes the high halves: buffer_store_short_d16_hi writes
+    // bits 16-31 of the register as they are (no shift), through a
+    // descriptor of the same form as make_buffer_rsrc's
+    const uint64_t yb = (uint64_t)(Ya + ((uint64_t)1 << k));
+    v4i rb;
+    rb.x = (int)(uint32_t)yb;
+    rb.y = (int)(uint32_t)(yb >> 32);
+    rb.z = (int)(2u << k);
+    rb.w = 0x00020000;
     const uint32_t wu = __builtin_amdgcn_readfirstlane(w);
     const int voff = (int)((G * 64 + l) * 2);
 #pragma unroll
@@ -458,7 +476,10 @@ __global__ __launch_bounds__(kGen14Threads, 2) void spec_gen14p(int16_t *__restr
                 const int soff = (int)(sb * X * 128u);                              // bytes of sb * X rows of 64
                 const uint32_t x = v[32 * hr + 8 * q + rr];
                 __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(x & 0xFFFFu), ra, voff, soff, 0);
-                __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(x >> 16), rb, voff, soff, 0);
+                asm volatile("buffer_store_short_d16_hi %0, %1, %2, %3 offen"
+                             :
+                             : "v"(x), "v"(voff), "s"(rb), "s"(soff)
+                             : "memory");
             }
 }
 
@@ -470,7 +491,7 @@ __global__ __launch_bounds__(kGen14Threads, 2) void spec_gen14p(int16_t *__restr
 // stores registers 16 p .. 16 p + 15 and gathers register 16 p + g of all 16
 // groups; the two groups of a wave store to opposite 32-dword halves of a row,
 // alternating with the register index, so stores and gathers are conflict
-// free), and sums F^2 by weight |s| + |S_lo| + |S_hi| as spec_last does.
+// free), and sums F^2 / 4 by weight |s| + |S_lo| + |S_hi| as spec_last sums F^2.
 // Tiles are visited so that the two column halves of a row segment go to
 // workgroups b and b + 8, which run on the same XCD (workgroups are dealt to
 // the 8 XCDs in turn) at the same time.
@@ -521,7 +542,10 @@ __global__ __launch_bounds__(kLast10Threads) void spec_last10(const int16_t *__r
                     v[q] = a + b;
                     v[q + hb] = a - b;
                 }
-        // |F| <= 2^24: a thread's 64 squares stay below 2^54 in one register pair
+        // v = Q: F / 2 = -Q, except at S = 0 (thread 0, register 0 of the
+        // tiles with sb = 0 and half = 0), where F / 2 = 2^23 - Q
+        if (sb == 0 && half == 0 && t == 0) v[0] -= 1 << 23;
+        // |F / 2| <= 2^23: a thread's 64 squares stay below 2^52 in one register pair
         unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int q = 0; q < 64; ++q) {

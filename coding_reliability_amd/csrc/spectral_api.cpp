@@ -132,6 +132,9 @@ cr_status spectral_finish(const SpectralPlan &p, const uint64_t *sl, uint64_t *r
             const u128 lo = (u128)e[0] + ((u128)e[1] << 64), hi = (u128)e[2] + ((u128)e[3] << 64);
             W[w] += lo + (hi << 32);
         }
+    // the int16-pair passes sum F^2 / 4 (they transform the parities, not the signs)
+    if (p.layout == kSpectralInt16Pairs)
+        for (uint32_t w = 0; w < p.nw; ++w) W[w] *= 4;
     // D(d) = 2^-n sum_w K_d(w) W(w); partial sums stay below C(n, d) 2^(n+k) <= 2^125
     memset(raw, 0, nbins * sizeof(uint64_t));
     for (uint32_t d = 0; d < p.nw; ++d) {
