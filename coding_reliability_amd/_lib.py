@@ -139,8 +139,15 @@ def _one_hip_runtime() -> None:
     torch is installed its runtime is loaded first (without importing torch);
     libcodrel.so then binds to it by soname, and a later `import torch`
     reuses the same file.  Device pointers and streams from torch and from
-    this library are then valid in each other's calls."""
+    this library are then valid in each other's calls.  A process that has a
+    HIP runtime mapped already keeps it: libcodrel.so binds to that one."""
     import importlib.util
+    try:
+        with open("/proc/self/maps") as maps:
+            if any("libamdhip64" in line for line in maps):
+                return
+    except OSError:
+        pass
     try:
         spec = importlib.util.find_spec("torch")
     except (ImportError, ValueError):

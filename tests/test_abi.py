@@ -101,3 +101,30 @@ def test_library_then_torch_share_one_hip_runtime():
     paths = r.stdout.split()
     assert sum("libamdhip64" in p for p in paths) == 1, paths
     assert sum("libhsa-runtime64" in p for p in paths) == 1, paths
+
+
+ROCM_FIRST = r"""
+import ctypes
+ctypes.CDLL('/opt/rocm/lib/libamdhip64.so.7', mode=ctypes.RTLD_GLOBAL)
+from coding_reliability_amd._lib import lib
+lib()
+paths = set()
+for line in open('/proc/self/maps'):
+    p = line.split()[-1]
+    if 'libamdhip64' in p:
+        paths.add(p)
+print('\n'.join(sorted(paths)))
+"""
+
+
+def test_library_keeps_a_runtime_already_mapped():
+    # a process that already holds a HIP runtime (here ROCm's) keeps it; the
+    # loader does not add torch's beside it
+    import subprocess
+    import sys
+    if not os.path.exists("/opt/rocm/lib/libamdhip64.so.7"):
+        pytest.skip("no ROCm HIP runtime at /opt/rocm/lib")
+    r = subprocess.run([sys.executable, "-c", ROCM_FIRST], capture_output=True, text=True, cwd=ROOT, timeout=300)
+    assert r.returncode == 0, r.stderr
+    paths = r.stdout.split()
+    assert len(paths) == 1 and "/opt/rocm" in paths[0], paths
