@@ -360,27 +360,28 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
 }
 
 // ---------------------------------------------------------------------------
-// k = 24 through int16 (2 bytes of traffic per element each way): the first
-// pass transforms index bits 0-13 (values within +-2^14 fit int16) of two
-// chunks at once, the last pass the remaining 10 bits.
+// k = 22 .. 24 through int16 (2 bytes of traffic per element each way): the
+// first pass transforms index bits 0 .. B1-1, B1 = k - 10 (values within
+// +-2^B1 fit int16), of two chunks at once, the last pass the remaining 10
+// bits.
 //
 // The passes transform the parities q(x) = parity(s & h(x)) in {0, 1} rather
 // than the signs (-1)^q = 1 - 2q: the transform of 1 - 2q is 2^k delta(S) -
 // 2 Q(S), Q the transform of q, so F^2 / 4 = Q^2 except at S = 0, where it is
 // (Q(0) - 2^(k-1))^2.  The last pass sums F^2 / 4 and the host multiplies the
-// weight sums by 4.  After 14 stages |Q| <= 2^14, within int16.
+// weight sums by 4.  After B1 <= 14 stages |Q| <= 2^14, within int16.
 //
-// spec_gen14p: workgroup (G, y) of 4 waves takes the 2^14 elements x_lo =
-// G * 2^14 + e of chunks sa = s0 + 2y (even) and sa + 1, each dword of its
-// working set holding the two chunks' values as int16 halves (packed
-// v_pk_add_u16 / v_pk_sub_i16 butterflies: one instruction per two values).
-// Since sa + 1 differs from sa in bit 0 only, parity((sa + 1) & h) =
-// parity(sa & h) ^ (h & 1).  Thread (wave w, lane l) generates e = w * 4096
-// + l * 64 + j (j < 64): bits 0-5 in registers; bits 6-11 after a transpose
-// within the wave through LDS (rows of 65 dwords: conflict free both ways);
-// bits 12-13 after an exchange across the waves in two rounds of 32
-// registers.  Each chunk's result is written as int16 in the
-// layout Y[c][S_lo >> 6][x_hi][S_lo & 63] (x_hi < 2^(k-14)), so that a tile of
+// spec_gen_pairs<W>: workgroup (G, y) of W waves (W = 2^(B1-12)) takes the
+// 2^B1 elements x_lo = G * 2^B1 + e of chunks sa = s0 + 2y (even) and sa + 1,
+// each dword of its working set holding the two chunks' values as int16
+// halves (packed v_pk_add_u16 / v_pk_sub_i16 butterflies: one instruction per
+// two values).  Since sa + 1 differs from sa in bit 0 only, parity((sa + 1) &
+// h) = parity(sa & h) ^ (h & 1).  Thread (wave w, lane l) generates e = w *
+// 4096 + l * 64 + j (j < 64): bits 0-5 in registers; bits 6-11 after a
+// transpose within the wave through LDS (rows of 65 dwords: conflict free
+// both ways); bits 12 .. B1-1 after an exchange across the waves in two
+// rounds of 32 registers.  Each chunk's result is written as int16 in the
+// layout Y[c][S_lo >> 6][x_hi][S_lo & 63] (x_hi < 1024), so that a tile of
 // the last pass -- all x_hi of 32 consecutive S_lo -- is one contiguous run of
 // 64-byte rows.
 typedef short v2s __attribute__((ext_vector_type(2)));
@@ -405,15 +406,17 @@ __device__ __forceinline__ void wht_pk(uint32_t (&v)[64]) {
             }
 }
 
-constexpr uint32_t kGen14Threads = 256;
-__global__ __launch_bounds__(kGen14Threads, 2) void spec_gen14p(int16_t *__restrict__ Y, uint32_t k, uint32_t A,
-                                                                 uint32_t Ainv, uint32_t tc, uint32_t s0) {
-    __shared__ uint32_t L[4 * 64 * 65];  // 65 KB: a 64 x 64 transpose per wave (rows of 65), then the exchange
+template <int W>
+__global__ __launch_bounds__(64 * W, 2) void spec_gen_pairs(int16_t *__restrict__ Y, uint32_t k, uint32_t A,
+                                                            uint32_t Ainv, uint32_t tc, uint32_t s0) {
+    constexpr uint32_t B1 = W == 4 ? 14 : (W == 2 ? 13 : 12);  // index bits of this pass
+    constexpr int QB = 32 / W;                                  // registers per wave and exchange round
+    __shared__ uint32_t L[W * 64 * 65];  // a 64 x 64 transpose per wave (rows of 65), then the exchange
     const uint32_t t = threadIdx.x, l = t & 63, w = t >> 6;
     const uint32_t mask = (1u << k) - 1;
     const uint32_t G = blockIdx.x;
     const uint32_t sa = s0 + 2 * blockIdx.y;
-    const uint32_t x0 = (G << 14) | (w << 12) | (l << 6);
+    const uint32_t x0 = (G << B1) | (w << 12) | (l << 6);
     const uint32_t i = (uint32_t)((uint64_t)x0 * Ainv) & mask;
     uint32_t h = (uint32_t)(((uint64_t)A * i) >> k);
     uint32_t v[64];
@@ -436,23 +439,25 @@ __global__ __launch_bounds__(kGen14Threads, 2) void spec_gen14p(int16_t *__restr
 #pragma unroll
     for (int r = 0; r < 64; ++r) v[r] = R[(uint32_t)r * 65 + l];
     wht_pk<1, 64>(v);  // bits 6-11 (v[r]: bits 6-11 = r, bits 0-5 = l); |v| <= 2^12
-    // bits 12-13: after round hr, v[32 hr + 8 q + rr] holds register
-    // 32 hr + 8 w + rr of wave q
+    if (W > 1) {
+        // bits 12 .. B1-1 (the wave index): after round hr, v[32 hr + QB q +
+        // rr] holds register 32 hr + QB w + rr of wave q
 #pragma unroll
-    for (int hr = 0; hr < 2; ++hr) {
-        __syncthreads();
+        for (int hr = 0; hr < 2; ++hr) {
+            __syncthreads();
 #pragma unroll
-        for (int rr = 0; rr < 32; ++rr) L[(w * 32 + rr) * 64 + l] = v[32 * hr + rr];
-        __syncthreads();
+            for (int rr = 0; rr < 32; ++rr) L[(w * 32 + rr) * 64 + l] = v[32 * hr + rr];
+            __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+            for (int q = 0; q < W; ++q)
 #pragma unroll
-            for (int rr = 0; rr < 8; ++rr) v[32 * hr + 8 * q + rr] = L[(q * 32 + 8 * w + rr) * 64 + l];
+                for (int rr = 0; rr < QB; ++rr) v[32 * hr + QB * q + rr] = L[(q * 32 + QB * w + rr) * 64 + l];
+        }
+        wht_pk<QB, 32>(v);  // over q; |v| <= 2^B1
     }
-    wht_pk<8, 32>(v);  // over q; |v| <= 2^14
     // stores through one buffer resource per chunk (2^k int16): the lane part
     // of the offset in a VGPR, the rest wave-uniform
-    const uint32_t X = 1u << (k - 14);
+    const uint32_t X = 1u << (k - B1);
     int16_t *Ya = Y + ((uint64_t)(2 * blockIdx.y) << k);
     const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(Ya, 0, (int)(2u << k), 0x00020000);
     // chunk sa + 1 takes the high halves: buffer_store_short_d16_hi writes
@@ -469,12 +474,12 @@ __global__ __launch_bounds__(kGen14Threads, 2) void spec_gen14p(int16_t *__restr
 #pragma unroll
     for (int hr = 0; hr < 2; ++hr)
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < W; ++q)
 #pragma unroll
-            for (int rr = 0; rr < 8; ++rr) {
-                const uint32_t sb = ((uint32_t)q << 6) | (32u * hr + 8u * wu + rr);  // S_lo >> 6
-                const int soff = (int)(sb * X * 128u);                              // bytes of sb * X rows of 64
-                const uint32_t x = v[32 * hr + 8 * q + rr];
+            for (int rr = 0; rr < QB; ++rr) {
+                const uint32_t sb = ((uint32_t)q << 6) | (32u * hr + QB * wu + rr);  // S_lo >> 6
+                const int soff = (int)(sb * X * 128u);                             // bytes of sb * X rows of 64
+                const uint32_t x = v[32 * hr + QB * q + rr];
                 __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(x & 0xFFFFu), ra, voff, soff, 0);
                 asm volatile("buffer_store_short_d16_hi %0, %1, %2, %3 offen"
                              :
@@ -506,13 +511,14 @@ __global__ __launch_bounds__(kLast10Threads) void spec_last10(const int16_t *__r
     const uint32_t t = threadIdx.x, cc = t & 31, g = t >> 5;
     for (uint32_t b = t; b < 64 * 2 * 32; b += kLast10Threads) bins[b] = 0;
     __syncthreads();
-    const uint32_t X = 1u << (k - 14);  // rows x_hi: 1024 at k = 24
-    const uint64_t ntiles = (uint64_t)nchunks * 512;
+    const uint32_t X = 1024;                  // rows x_hi: the first pass took index bits 0 .. k-11
+    const uint32_t sbits = k - 16;            // bits of sb
+    const uint64_t ntiles = (uint64_t)nchunks << (k - 15);
     for (uint64_t u = blockIdx.x; u < ntiles; u += gridDim.x) {
         const uint64_t jx = u >> 3;
         const uint32_t half = (uint32_t)(jx & 1);
         const uint64_t P = ((jx >> 1) << 3) | (u & 7);
-        const uint32_t c = (uint32_t)(P >> 8), sb = (uint32_t)(P & 255);
+        const uint32_t c = (uint32_t)(P >> sbits), sb = (uint32_t)(P & ((1u << sbits) - 1));
         const int16_t *tile = Y + ((uint64_t)c << k) + (uint64_t)sb * X * 64 + half * 32;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<int16_t *>(tile), 0, (int)(((X - 1) * 64 + 32) * 2), 0x00020000);
@@ -543,8 +549,8 @@ __global__ __launch_bounds__(kLast10Threads) void spec_last10(const int16_t *__r
                     v[q + hb] = a - b;
                 }
         // v = Q: F / 2 = -Q, except at S = 0 (thread 0, register 0 of the
-        // tiles with sb = 0 and half = 0), where F / 2 = 2^23 - Q
-        if (sb == 0 && half == 0 && t == 0) v[0] -= 1 << 23;
+        // tiles with sb = 0 and half = 0), where F / 2 = 2^(k-1) - Q
+        if (sb == 0 && half == 0 && t == 0) v[0] -= 1 << (k - 1);
         // |F / 2| <= 2^23: a thread's 64 squares stay below 2^52 in one register pair
         unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -662,13 +668,21 @@ cr_status launch_spectral_batch(void *X, uint32_t layout, uint32_t k, uint64_t A
     if ((A >> 32) || (tc >> 32) || (s0 + nchunks) > (1ull << 32))
         return fail(CR_ERR_UNSUPPORTED, "spectral path: multiplier or transform count beyond 32 bits");
     if (layout == kSpectralInt16Pairs) {
-        // X: nchunks * 2^k int16 after index bits 0-13, in pairs of transforms
-        if (k != 24 || (nchunks & 1) || (s0 & 1) || (nslots & 15))
+        // X: nchunks * 2^k int16 after index bits 0 .. k-11, in pairs of transforms
+        if (k < 22 || k > 24 || (nchunks & 1) || (s0 & 1) || (nslots & 15))
             return fail(CR_ERR_INVALID_ARGUMENT, "spectral int16 pairs: k=%u, %u transforms from %llu, %u slots", k,
                         nchunks, (unsigned long long)s0, nslots);
         int16_t *y = static_cast<int16_t *>(X);
-        hipLaunchKernelGGL(spec_gen14p, dim3(1u << (k - 14), nchunks / 2), dim3(kGen14Threads), 0, stream, y, k,
-                           (uint32_t)A, Ainv, (uint32_t)tc, (uint32_t)s0);
+        const dim3 grid(1024, nchunks / 2);
+        if (k == 24)
+            hipLaunchKernelGGL(spec_gen_pairs<4>, grid, dim3(256), 0, stream, y, k, (uint32_t)A, Ainv, (uint32_t)tc,
+                               (uint32_t)s0);
+        else if (k == 23)
+            hipLaunchKernelGGL(spec_gen_pairs<2>, grid, dim3(128), 0, stream, y, k, (uint32_t)A, Ainv, (uint32_t)tc,
+                               (uint32_t)s0);
+        else
+            hipLaunchKernelGGL(spec_gen_pairs<1>, grid, dim3(64), 0, stream, y, k, (uint32_t)A, Ainv, (uint32_t)tc,
+                               (uint32_t)s0);
         CR_HIP(hipGetLastError());
         hipLaunchKernelGGL(spec_last10, dim3(nslots), dim3(kLast10Threads), 0, stream, (const int16_t *)y, k, s0,
                            nchunks, slots, nbins);

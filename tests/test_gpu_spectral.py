@@ -49,6 +49,16 @@ def test_narrow_last_pass_matches_pair_kernel(ctx, A, k):
     assert raw_s == raw_p
 
 
+@pytest.mark.parametrize("A,k", [(58659, 22), (3, 22), (58659, 23), (1021, 23)])
+def test_int16_pairs_k22_k23_match_pair_kernel(ctx, A, k):
+    # k = 22 / 23 run the int16-pair passes with a first pass of 12 / 13 bits
+    # (one / two waves per workgroup) and the 10-bit last pass
+    raw_s, ms_s = ctx.an_pairs_spectral(A, k)
+    raw_p, ms_p = ctx.an_pairs(A, 1 << k)
+    print(f"A={A} k={k}: spectral {ms_s:.1f} ms, pair kernel {ms_p:.1f} ms", flush=True)
+    assert raw_s == raw_p
+
+
 def test_k25_multi_transform_matches_pair_kernel(ctx):
     # A = 61 at k = 25: 31-bit codewords, 64 transforms of 2^25 through the
     # middle pass and the narrow last pass (per-transform weight base and high
