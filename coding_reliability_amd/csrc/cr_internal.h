@@ -180,9 +180,9 @@ struct SpectralPlan {
     double sec_per_element = 0;   // measured device seconds per transformed element
 };
 // Pass structures of the spectral path: index bits 0-15 (or 0-7 for k <= 16)
-// then the rest through an int32 (int16) buffer; or, for k = 24 and at least
-// two transforms, bits 0-13 of two transforms at once then bits 14-23,
-// through int16.
+// then the rest through an int32 (int16) buffer; or, for k = 22 .. 24 and at
+// least two transforms, bits 0 .. k-11 of two transforms at once then the
+// last 10 bits, through int16.
 constexpr uint32_t kSpectralInt32 = 0, kSpectralInt16Pairs = 1;
 // False when (A, k) is outside the spectral path (then the pair kernel applies).
 bool spectral_plan(uint64_t A, uint32_t k, int cus, SpectralPlan *p);
