@@ -102,15 +102,18 @@ TESTS_CPP = os.path.join(ROOT, "tests", "cpp")
 
 
 def build_test_programs() -> list[str]:
-    """C++ callers of the reference entry points used by the GPU tests
-    (tests/cpp/*.cpp, compiled against include/cr_reference_api.h and linked
-    against libcodrel.so), into _build/tests/."""
+    """Callers of the library used by the GPU tests, into _build/tests/: the
+    C++ ones of the reference entry points (tests/cpp/*.cpp, against
+    include/cr_reference_api.h) and the plain C ones of the C ABI
+    (tests/cpp/*.c, C99 against include/coding_reliability.h), each linked
+    against libcodrel.so."""
     out_dir = os.path.join(BUILD, "tests")
     os.makedirs(out_dir, exist_ok=True)
     out = []
-    for src in sorted(glob.glob(os.path.join(TESTS_CPP, "*.cpp"))):
+    for src in sorted(glob.glob(os.path.join(TESTS_CPP, "*.cpp")) + glob.glob(os.path.join(TESTS_CPP, "*.c"))):
         exe = os.path.join(out_dir, os.path.splitext(os.path.basename(src))[0])
-        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src, "-L", PKG,
+        lang = ["gcc", "-std=c99"] if src.endswith(".c") else ["g++", "-std=c++17"]
+        cmd = [*lang, "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), src, "-L", PKG,
                "-lcodrel", "-Wl,-rpath,$ORIGIN/../../coding_reliability_amd", "-o", exe]
         subprocess.run(cmd, check=True)
         out.append(exe)

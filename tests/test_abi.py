@@ -128,3 +128,14 @@ def test_library_keeps_a_runtime_already_mapped():
     assert r.returncode == 0, r.stderr
     paths = r.stdout.split()
     assert len(paths) == 1 and "/opt/rocm" in paths[0], paths
+
+
+def test_c_header_is_c99():
+    # the C ABI header compiles as C99 (no C++ in its declarations)
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("no gcc")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-x", "c",
+                        os.path.join(ROOT, "include", "coding_reliability.h")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
