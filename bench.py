@@ -273,8 +273,8 @@ def run_kernels(ctx, ceil: dict) -> dict:
     assert sum(h) == 1 << k
     out["hamming_exhaustive_39_32"] = _kernel_entry("hamming_exhaustive_39_32", 1 << k, ms, "Gwords/s", ceil,
                                                     "(39,32) weights of all 2^32 data words")
-    h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 20, stride=65537))
-    assert sum(h) == 1 << 20
+    h_grid, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_STRIDE, 0, 1 << 20, stride=65537))
+    assert sum(h_grid) == 1 << 20
     e = _kernel_entry("hamming_grid_39_32", 1 << 20, ms, "Gwords/s", ceil,
                       "(39,32) integer grid d = i*65537 mod 2^32, i < 2^20")
     # 2^20 words are well under a microsecond of device work: the launch is
@@ -286,6 +286,24 @@ def run_kernels(ctx, ceil: dict) -> dict:
     h, ms = _warm(lambda: ctx.hamming_weights(k, cra.HAM_MC, 0, S, seed=2026, stream=0))
     assert sum(h) == S
     out["hamming_mc_39_32"] = _kernel_entry("hamming_mc_39_32", S, ms, "Gwords/s", ceil, "(39,32) 2^34 Philox words")
+    # the grid as a second job of the Monte Carlo launch (cr_hamming_weights_batch):
+    # it then costs its words, not a launch.  The Monte Carlo launch alone
+    # varies by a few percent from call to call, so five alternating pairs
+    # (alone, with the grid) are timed and the median difference reported
+    reqs = [dict(k=k, source=cra.HAM_STRIDE, count=1 << 20, stride=65537),
+            dict(k=k, source=cra.HAM_MC, count=S, seed=2026)]
+    rows, _ = _warm(lambda: ctx.hamming_weights_batch(reqs))
+    assert rows == [h_grid, h]
+    pairs = []
+    for _ in range(5):
+        alone = ctx.hamming_weights(k, cra.HAM_MC, 0, S, seed=2026, stream=0)[1]
+        pairs.append((alone, ctx.hamming_weights_batch(reqs)[1]))
+    diffs = sorted(b - a for a, b in pairs)
+    e["with_mc_2pow34_one_launch"] = {
+        "ms_median": round(sorted(b for _, b in pairs)[2], 4),
+        "mc_alone_ms_median": round(sorted(a for a, _ in pairs)[2], 4),
+        "grid_marginal_ms_median": round(diffs[2], 4),
+        "grid_marginal_ms_range": [round(diffs[0], 4), round(diffs[-1], 4)]}
     S = 1 << 30
     h, ms = _warm(lambda: ctx.hamming_weights(64, cra.HAM_MC, 0, S, seed=2026, stream=1))
     assert sum(h) == S

@@ -267,6 +267,30 @@ class Context:
         check(lib().cr_hamming_weights(self._h, C.byref(req), _u64p(out), nbins, C.byref(ms)))
         return [int(x) for x in out], ms.value
 
+    def hamming_weights_batch(self, requests) -> tuple[list[list[int]], float]:
+        """hamming_weights of several requests in one call
+        (cr_hamming_weights_batch): each request is a dict of hamming_weights'
+        keyword arguments (k required); row i has hamming_N(k_i) bins."""
+        reqs = (cr_ham_request * len(requests))()
+        keep = []
+        for r, q in zip(reqs, requests):
+            k, source = q["k"], q.get("source", HAM_RANGE)
+            lst, count = None, q.get("count")
+            if source == HAM_LIST:
+                lst = np.ascontiguousarray(q["words"], dtype=np.uint64)
+                keep.append(lst)
+                count = len(lst)
+            if count is None:
+                count = 1 << k
+            r.k, r.source, r.first, r.count, r.stride = k, source, q.get("first", 0), count, q.get("stride", 0)
+            r.list = _u64p(lst) if lst is not None else None
+            r.seed, r.stream = q.get("seed", 0), q.get("stream", 0)
+        nbins = max((hamming_N(q["k"]) for q in requests), default=1)
+        out = np.zeros((len(requests), nbins), dtype=np.uint64)
+        ms = C.c_double(0)
+        check(lib().cr_hamming_weights_batch(self._h, reqs, len(requests), _u64p(out), nbins, C.byref(ms)))
+        return [[int(x) for x in row[: hamming_N(q["k"])]] for row, q in zip(out, requests)], ms.value
+
     def an_mc(self, A: int, n: int, m: int, seed: int, nbins: int = 65) -> tuple[list[int], float]:
         """Monte Carlo (word, sample) histogram of run_ancoding_mc with Philox samples."""
         out = np.zeros(nbins, dtype=np.uint64)

@@ -101,6 +101,8 @@ SIGNATURES = {
     "cr_hamming_parity_bits": (C.c_uint32, [C.c_uint32]),
     "cr_hamming_weights": (C.c_int, [C.c_void_p, C.POINTER(cr_ham_request), u64p, C.c_uint32,
                                      C.POINTER(C.c_double)]),
+    "cr_hamming_weights_batch": (C.c_int, [C.c_void_p, C.POINTER(cr_ham_request), C.c_uint32, u64p, C.c_uint32,
+                                           C.POINTER(C.c_double)]),
     "cr_hamming_report": (C.c_int, [C.c_uint32, u64p, C.c_int, u64p]),
     "cr_hamming_sdc_table": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int, u64p, C.c_uint32, C.POINTER(C.c_void_p)]),
     "cr_an_batch": (C.c_int, [C.c_void_p, C.POINTER(cr_an_code), C.c_uint32, u64p, C.c_uint32,

@@ -164,6 +164,7 @@ struct HamJob {
 };
 static_assert(sizeof(HamJob) == 152, "HamJob layout");
 constexpr uint32_t kHamMaxLaneSamples = 4096;
+constexpr uint32_t kHamMaxBatch = 4096;  // requests per cr_hamming_weights_batch call
 
 // True when num_words = 2^k and cr_an_pairs_spectral supports (A, k) and is
 // estimated at under 80 % of the pair kernel's time (measured rates:

@@ -34,99 +34,178 @@ extern "C" uint32_t cr_hamming_parity_bits(uint32_t k) {
     return hamming_masks(k, m);
 }
 
-extern "C" cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *req, uint64_t *hist, uint32_t nbins,
-                                        double *kernel_ms) {
-    clear_error();
-    if (!ctx || !req || !hist) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
-    if (req->k < 1 || req->k > 64) return fail(CR_ERR_INVALID_ARGUMENT, "k=%u outside [1, 64]", req->k);
-    if (req->source > CR_HAM_MC) return fail(CR_ERR_INVALID_ARGUMENT, "unknown source %u", req->source);
-    if (req->source == CR_HAM_LIST && req->count && !req->list)
-        return fail(CR_ERR_INVALID_ARGUMENT, "list source without a list");
-    const uint32_t per = req->k <= 32 ? 4 : 2;
-    if (req->source == CR_HAM_MC && req->first % per)
-        return fail(CR_ERR_INVALID_ARGUMENT, "Monte Carlo first sample must be a multiple of %u", per);
-    HamJob job{};
-    const uint32_t r = hamming_masks(req->k, job.masks);
-    const uint32_t need = req->k + r + 1;  // weights 0..k+r
-    if (nbins < need) return fail(CR_ERR_INVALID_ARGUMENT, "nbins=%u but weights reach %u", nbins, need - 1);
+namespace {
+
+// Requests [0, n) as one Hamming job each: the jobs of 32-bit words in one
+// launch, those of 64-bit words (k > 32) in a second on the same stream, each
+// launch's jobs in request order; requests without samples launch nothing.
+// hists[i * nbins + b] is request i's count of weight b.
+cr_status hamming_run(cr_context *ctx, const cr_ham_request *reqs, uint32_t n, uint64_t *hists, uint32_t nbins,
+                      double *kernel_ms) {
+    if (!ctx || !reqs || !hists) return fail(CR_ERR_INVALID_ARGUMENT, "NULL argument");
+    if (n == 0 || n > kHamMaxBatch)
+        return fail(CR_ERR_INVALID_ARGUMENT, "%u requests outside [1, %u]", n, kHamMaxBatch);
+    struct Launch {
+        std::vector<HamJob> jobs;
+        std::vector<uint32_t> index, needs;  // request of each job, and its weight bins
+        std::vector<double> rates;
+        double est_s = 0;
+        uint32_t rows = 2;
+        uint64_t items = 0;
+        bool short_launch = false;
+    } launches[2];
+    uint64_t list_words = 0;
+    for (uint32_t q = 0; q < n; ++q) {
+        const cr_ham_request *req = reqs + q;
+        if (req->k < 1 || req->k > 64) return fail(CR_ERR_INVALID_ARGUMENT, "k=%u outside [1, 64]", req->k);
+        if (req->source > CR_HAM_MC) return fail(CR_ERR_INVALID_ARGUMENT, "unknown source %u", req->source);
+        if (req->source == CR_HAM_LIST && req->count && !req->list)
+            return fail(CR_ERR_INVALID_ARGUMENT, "list source without a list");
+        const uint32_t per = req->k <= 32 ? 4 : 2;
+        if (req->source == CR_HAM_MC && req->first % per)
+            return fail(CR_ERR_INVALID_ARGUMENT, "Monte Carlo first sample must be a multiple of %u", per);
+        HamJob job{};
+        const uint32_t r = hamming_masks(req->k, job.masks);
+        const uint32_t need = req->k + r + 1;  // weights 0..k+r
+        if (nbins < need) return fail(CR_ERR_INVALID_ARGUMENT, "nbins=%u but weights reach %u", nbins, need - 1);
+        if (req->count == 0) continue;
+        job.first = req->first;
+        job.stride = req->stride;
+        job.count = req->count;
+        job.seed = req->seed;
+        job.source = req->source;
+        job.k = req->k;
+        job.stream = req->stream;
+        job.nbins = need;
+        // Measured rates (words/s): exhaustive 7.5e12, 32-bit range/stride
+        // 1.9e12, 64-bit 0.55e12, Monte Carlo 1.55e12 / 0.75e12, lists about
+        // half the stride rates.
+        double rate = req->k > 32 ? 0.55e12 : 1.9e12;
+        if (req->source == CR_HAM_RANGE && req->first == 0) rate = 7.5e12;
+        if (req->source == CR_HAM_MC) rate = req->k > 32 ? 0.75e12 : 1.55e12;
+        if (req->source == CR_HAM_LIST) {
+            rate *= 0.5;
+            list_words += req->count;
+        }
+        Launch &L = launches[req->k > 32];
+        L.est_s += (double)req->count / rate;
+        L.rows = std::max(L.rows, need + 1);
+        L.jobs.push_back(job);
+        L.index.push_back(q);
+        L.needs.push_back(need);
+        L.rates.push_back(rate);
+    }
     CR_HIP(hipSetDevice(ctx->device));
-    memset(hist, 0, nbins * sizeof(uint64_t));
-    if (req->count == 0) {
-        if (kernel_ms) *kernel_ms = 0;
-        return CR_OK;
+    memset(hists, 0, (size_t)n * nbins * sizeof(uint64_t));
+    if (kernel_ms) *kernel_ms = 0;
+    if (launches[0].jobs.empty() && launches[1].jobs.empty()) return CR_OK;
+    if (list_words) CR_TRY(ctx->table_a.reserve(list_words * sizeof(uint64_t)));
+    uint64_t list_at = 0, entries = 0;
+    std::vector<uint64_t> out_at[2];
+    for (Launch &L : launches) {
+        // Work assignment.  The queue balances waves that advance at different
+        // rates (a static assignment measured about 25 % slower on launches
+        // of 0.5 ms and more), but each queue step is a device-scope atomic
+        // on one counter, serialized at about 12.5 ns, one per item plus one
+        // per wave: 40-150 us per launch.  Launches estimated below 100 us of
+        // work therefore run with a static assignment.
+        L.short_launch = L.est_s < 100e-6;
+        const uint64_t waves = (uint64_t)hamming_waves_per_cu(L.rows) * ctx->cus;
+        std::vector<uint64_t> &at = out_at[&L - launches];
+        at.resize(L.jobs.size());
+        for (size_t j = 0; j < L.jobs.size(); ++j) {
+            HamJob &job = L.jobs[j];
+            // items of 64 x spl words, spl the smallest power of two in [16,
+            // kHamMaxLaneSamples] that gives at most 8 items per resident
+            // wave for a static assignment or 32 for the queue (the last
+            // items set the launch's tail: with at most 4 per wave a
+            // 2^30-word (72,64) Monte Carlo launch ran 20 % below the rate of
+            // 2^34 words), and for the queue at least rate/8e7 words per item
+            // (the counter's 8e7 items/s)
+            const uint64_t per_wave = L.short_launch ? 8 : 32;
+            const uint64_t want = (job.count + 64 * per_wave * waves - 1) / (64 * per_wave * waves);
+            const double min_words = L.short_launch ? 0.0 : L.rates[j] / 8e7;
+            uint64_t spl = 16;
+            while ((spl < want || 64.0 * (double)spl < min_words) && spl < kHamMaxLaneSamples) spl *= 2;
+            job.lane_samples = (uint32_t)spl;
+            job.items = hamming_items(job.count, job.lane_samples);
+            job.item_begin = L.items;
+            L.items += job.items;
+            if (job.source == kHamList) {
+                const cr_ham_request *req = reqs + L.index[j];
+                CR_HIP(hipMemcpyAsync(ctx->table_a.as<uint64_t>() + list_at, req->list,
+                                      req->count * sizeof(uint64_t), hipMemcpyHostToDevice, ctx->stream));
+                job.list = ctx->table_a.as<uint64_t>() + list_at;
+                list_at += req->count;
+            }
+            // dev::kHistCopies line-aligned copies of each histogram, summed
+            // on the host
+            job.out_copies = dev::kHistCopies;
+            at[j] = entries;
+            entries += dev::hist_copy_stride(L.needs[j]) * job.out_copies;
+        }
     }
-    job.first = req->first;
-    job.stride = req->stride;
-    job.count = req->count;
-    job.seed = req->seed;
-    job.source = req->source;
-    job.k = req->k;
-    job.stream = req->stream;
-    job.nbins = need;
-    job.item_begin = 0;
-    // Work assignment.  The queue balances waves that advance at different
-    // rates (a static assignment measured about 25 % slower on launches of
-    // 0.5 ms and more), but each queue step is a device-scope atomic on one
-    // counter, serialized at about 12.5 ns, one per item plus one per wave:
-    // 40-150 us per launch.  Launches estimated below 100 us of work
-    // therefore run with a static assignment.  Measured rates (words/s):
-    // exhaustive 7.5e12, 32-bit range/stride 1.9e12, 64-bit 0.55e12, Monte
-    // Carlo 1.55e12 / 0.75e12, lists about half the stride rates.
-    double rate = req->k > 32 ? 0.55e12 : 1.9e12;
-    if (req->source == CR_HAM_RANGE && req->first == 0) rate = 7.5e12;
-    if (req->source == CR_HAM_MC) rate = req->k > 32 ? 0.75e12 : 1.55e12;
-    if (req->source == CR_HAM_LIST) rate *= 0.5;
-    const bool short_launch = (double)req->count / rate < 100e-6;
-    {
-        // items of 64 x spl words, spl the smallest power of two in [16,
-        // kHamMaxLaneSamples] that gives at most 8 items per resident wave for
-        // a static assignment or 16 for the queue (the last items set the
-        // launch's tail: with at most 4 per wave a 2^30-word (72,64) Monte
-        // Carlo launch ran 20 % below the rate of 2^34 words), and for the
-        // queue at least rate/8e7 words per item (the counter's 8e7 items/s)
-        const uint64_t waves = (uint64_t)hamming_waves_per_cu(need + 1) * ctx->cus;
-        const uint64_t per_wave = short_launch ? 8 : 32;
-        const uint64_t want = (req->count + 64 * per_wave * waves - 1) / (64 * per_wave * waves);
-        const double min_words = short_launch ? 0.0 : rate / 8e7;
-        uint64_t spl = 16;
-        while ((spl < want || 64.0 * (double)spl < min_words) && spl < kHamMaxLaneSamples) spl *= 2;
-        job.lane_samples = (uint32_t)spl;
-    }
-    job.items = hamming_items(req->count, job.lane_samples);
-    if (req->source == CR_HAM_LIST) {
-        CR_TRY(ctx->table_a.reserve(req->count * sizeof(uint64_t)));
-        CR_HIP(hipMemcpyAsync(ctx->table_a.ptr, req->list, req->count * sizeof(uint64_t), hipMemcpyHostToDevice,
-                              ctx->stream));
-        job.list = ctx->table_a.as<uint64_t>();
-    }
-    // dev::kHistCopies line-aligned copies of the histogram, summed here
-    job.out_copies = dev::kHistCopies;
-    const uint64_t cstride = dev::hist_copy_stride(need);
-    const uint64_t entries = cstride * job.out_copies;
+    const size_t njobs = launches[0].jobs.size() + launches[1].jobs.size();
     CR_TRY(ctx->hist.reserve(entries * sizeof(uint64_t)));
-    CR_TRY(ctx->counters.reserve(sizeof(uint64_t)));
-    CR_TRY(ctx->jobs.reserve(sizeof(HamJob)));
-    job.out = ctx->hist.as<unsigned long long>();
-    CR_TRY(fill_zero_u64(job.out, entries, ctx->stream));
-    CR_HIP(hipMemcpyAsync(ctx->jobs.ptr, &job, sizeof(job), hipMemcpyHostToDevice, ctx->stream));
-    CR_TRY(launch_hamming(ctx->jobs.as<HamJob>(), 1, job.items, req->k > 32, ctx->counters.as<unsigned long long>(),
-                          short_launch, need + 1, ctx->cus, ctx->stream, kernel_ms ? ctx->ev0 : nullptr,
-                          kernel_ms ? ctx->ev1 : nullptr));
+    CR_TRY(ctx->counters.reserve(2 * sizeof(uint64_t)));
+    CR_TRY(ctx->jobs.reserve(njobs * sizeof(HamJob)));
+    std::vector<HamJob> all;
+    all.reserve(njobs);
+    for (Launch &L : launches)
+        for (size_t j = 0; j < L.jobs.size(); ++j) {
+            L.jobs[j].out = ctx->hist.as<unsigned long long>() + out_at[&L - launches][j];
+            all.push_back(L.jobs[j]);
+        }
+    CR_TRY(fill_zero_u64(ctx->hist.as<unsigned long long>(), entries, ctx->stream));
+    CR_HIP(hipMemcpyAsync(ctx->jobs.ptr, all.data(), njobs * sizeof(HamJob), hipMemcpyHostToDevice, ctx->stream));
+    // HIP events before the first launch and after the last
+    const bool both = !launches[0].jobs.empty() && !launches[1].jobs.empty();
+    for (int w = 0, first = 1; w < 2; ++w) {
+        const Launch &L = launches[w];
+        if (L.jobs.empty()) continue;
+        const bool last = w == 1 || !both;
+        CR_TRY(launch_hamming(ctx->jobs.as<HamJob>() + (w ? launches[0].jobs.size() : 0), (uint32_t)L.jobs.size(),
+                              L.items, w == 1, ctx->counters.as<unsigned long long>() + w, L.short_launch, L.rows,
+                              ctx->cus, ctx->stream, kernel_ms && first ? ctx->ev0 : nullptr,
+                              kernel_ms && last ? ctx->ev1 : nullptr));
+        first = 0;
+    }
     CR_TRY(ctx->host.reserve(entries * sizeof(uint64_t)));
-    CR_HIP(hipMemcpyAsync(ctx->host.ptr, job.out, entries * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    CR_HIP(hipMemcpyAsync(ctx->host.ptr, ctx->hist.ptr, entries * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          ctx->stream));
     CR_HIP(hipStreamSynchronize(ctx->stream));
     const uint64_t *copies = static_cast<const uint64_t *>(ctx->host.ptr);
-    for (uint32_t b = 0; b < need; ++b) {
-        uint64_t s = 0;
-        for (uint32_t c = 0; c < job.out_copies; ++c) s += copies[c * cstride + b];
-        hist[b] = s;
-    }
+    for (const Launch &L : launches)
+        for (size_t j = 0; j < L.jobs.size(); ++j) {
+            const uint64_t cstride = dev::hist_copy_stride(L.needs[j]);
+            const uint64_t base = out_at[&L - launches][j];
+            uint64_t *hist = hists + (uint64_t)L.index[j] * nbins;
+            for (uint32_t b = 0; b < L.needs[j]; ++b) {
+                uint64_t sum = 0;
+                for (uint32_t c = 0; c < L.jobs[j].out_copies; ++c) sum += copies[base + c * cstride + b];
+                hist[b] = sum;
+            }
+        }
     if (kernel_ms) {
         float ms = 0;
         CR_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
         *kernel_ms = ms;
     }
     return CR_OK;
+}
+
+}  // namespace
+
+extern "C" cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *req, uint64_t *hist, uint32_t nbins,
+                                        double *kernel_ms) {
+    clear_error();
+    return hamming_run(ctx, req, 1, hist, nbins, kernel_ms);
+}
+
+extern "C" cr_status cr_hamming_weights_batch(cr_context *ctx, const cr_ham_request *reqs, uint32_t n,
+                                              uint64_t *hists, uint32_t nbins, double *kernel_ms) {
+    clear_error();
+    return hamming_run(ctx, reqs, n, hists, nbins, kernel_ms);
 }
 
 extern "C" cr_status cr_hamming_report(uint32_t k, const uint64_t *weights, int with_1bit, uint64_t *out_u128) {

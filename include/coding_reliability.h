@@ -209,6 +209,16 @@ uint32_t cr_hamming_parity_bits(uint32_t k);
 cr_status cr_hamming_weights(cr_context *ctx, const cr_ham_request *req, uint64_t *hist, uint32_t nbins,
                              double *kernel_ms);
 
+/* n requests (1..4096) in one call: hists[i * nbins + w] is request i's
+ * histogram, equal to cr_hamming_weights of that request; nbins must exceed
+ * every request's largest weight.  The requests of k <= 32 run as the jobs of
+ * one kernel launch, those of k > 32 as the jobs of a second; kernel_ms spans
+ * both.  No counterpart in the reference: it puts small requests (the (39,32)
+ * stride grid of 2^20 words, a few microseconds of work) into the launch of a
+ * larger one (its 2^34-word Monte Carlo run) instead of paying a launch each. */
+cr_status cr_hamming_weights_batch(cr_context *ctx, const cr_ham_request *reqs, uint32_t n, uint64_t *hists,
+                                   uint32_t nbins, double *kernel_ms);
+
 /* Post-processing of run_hamming (hamming.cu:182-208) from a non-extended
  * weight histogram: out[0] = 2^k, out[1] = N*2^k with spheres, odd weights
  * merged into the next even bin, the 1-bit-sphere closed form for odd bins,

@@ -55,6 +55,7 @@ def test_null_arguments_are_refused_without_gpu():
     assert L.cr_an_pairs(null, 61, 16, null, 0, null) == 1
     assert L.cr_an_batch(null, null, 0, null, 0, null, null) == 1
     assert L.cr_hamming_weights(null, null, null, 0, null) == 1
+    assert L.cr_hamming_weights_batch(null, null, 1, null, 0, null) == 1
     assert L.cr_an_candidates(null, 8, 5, null, null, 0, null, null) == 1
     assert L.cr_server_create(null, null, null) == 1
     assert L.cr_batch_plan_create(null, null, 0, 0, null, null) == 1

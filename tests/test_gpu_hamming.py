@@ -124,6 +124,51 @@ def test_mc_72_64_within_4sigma_macwilliams(ctx):
         assert abs(E[w] - S * p) <= 4 * math.sqrt(S * p * (1 - p)) + 1, w
 
 
+def test_batch_equals_separate_calls(ctx):
+    # one call: the (39,32) stride grid of 2^20 words beside its 2^34-word
+    # Monte Carlo run, an exhaustive (22,16) code, a list, an empty request
+    # and two 64-bit requests (the second launch); every row equals the
+    # request's own cr_hamming_weights call bit for bit
+    words = (np.arange(5000, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15)) >> np.uint64(40)
+    reqs = [dict(k=32, source=cra.HAM_STRIDE, count=1 << 20, stride=65537),
+            dict(k=32, source=cra.HAM_MC, count=1 << 34, seed=20261202),
+            dict(k=16),
+            dict(k=64, source=cra.HAM_MC, first=2, count=33333, seed=5, stream=1),
+            dict(k=24, source=cra.HAM_LIST, words=words),
+            dict(k=8, count=0),
+            dict(k=57, source=cra.HAM_RANGE, first=(1 << 56) - 999, count=1 << 20)]
+    rows, t_batch = ctx.hamming_weights_batch(reqs)
+    t_sep = 0.0
+    for q, row in zip(reqs, rows):
+        q = dict(q)
+        h, t = ctx.hamming_weights(q.pop("k"), **q)
+        t_sep += t
+        assert row == h, q
+    assert rows[5] == [0] * len(rows[5])
+    print(f"batch of {len(reqs)}: {t_batch:.2f} ms, separate calls {t_sep:.2f} ms", flush=True)
+
+
+def test_batch_c3_grid_in_mc_launch(ctx):
+    # the grid alone is launch-bound; in the Monte Carlo launch it costs its
+    # words only (checked against the oracle, not against another call)
+    grid = dict(k=32, source=cra.HAM_STRIDE, count=1 << 20, stride=65537)
+    rows, ms = ctx.hamming_weights_batch([grid, dict(k=32, source=cra.HAM_MC, count=1 << 30, seed=3)])
+    words = (np.arange(1 << 20, dtype=np.uint64) * np.uint64(65537)) & np.uint64(0xFFFFFFFF)
+    assert rows[0] == O.hamming_weights_list(32, words)[: len(rows[0])]
+    assert sum(rows[1]) == 1 << 30
+
+
+def test_batch_refusals(ctx):
+    with pytest.raises(cra.CRError):
+        ctx.hamming_weights_batch([])
+    with pytest.raises(cra.CRError):
+        ctx.hamming_weights_batch([dict(k=16), dict(k=65)])
+    with pytest.raises(cra.CRError):
+        ctx.hamming_weights_batch([dict(k=32, source=cra.HAM_MC, first=2, count=8)])
+    rows, ms = ctx.hamming_weights_batch([dict(k=16, count=0), dict(k=64, count=0)])  # nothing launched
+    assert ms == 0 and not any(rows[0]) and not any(rows[1])
+
+
 @pytest.mark.parametrize("A,n,m", [(61, 8, 16), (641, 10, 100), (58659, 12, 64), (3, 16, 5),
                                    (61, 12, 1024), (58659, 16, 512), (641, 14, 700), (65533, 9, 1280)])
 def test_an_mc_bitexact_vs_oracle(ctx, A, n, m):
