@@ -656,9 +656,47 @@ cr_status launch_narrow(uint32_t R, const In *X, uint32_t k, uint32_t tbit, uint
     return CR_OK;
 }
 
+template <class In>
+const void *narrow_kernel(uint32_t R) {
+    switch (R) {
+    case 1: return reinterpret_cast<const void *>(&spec_last_narrow<In, 1>);
+    case 2: return reinterpret_cast<const void *>(&spec_last_narrow<In, 2>);
+    case 3: return reinterpret_cast<const void *>(&spec_last_narrow<In, 3>);
+    case 4: return reinterpret_cast<const void *>(&spec_last_narrow<In, 4>);
+    case 5: return reinterpret_cast<const void *>(&spec_last_narrow<In, 5>);
+    case 6: return reinterpret_cast<const void *>(&spec_last_narrow<In, 6>);
+    case 7: return reinterpret_cast<const void *>(&spec_last_narrow<In, 7>);
+    default: return nullptr;
+    }
+}
+
 }  // namespace
 
 size_t spectral_tile_count(uint32_t k) { return (size_t)1 << (k - 14); }
+
+uint32_t spectral_last_pass_per_cu(uint32_t layout, uint32_t k) {
+    const void *f = nullptr;
+    int threads = (int)kThreads;
+    if (layout == kSpectralInt16Pairs) {
+        f = reinterpret_cast<const void *>(&spec_last10);
+        threads = (int)kLast10Threads;
+    } else if (k < 16) {
+        f = narrow_kernel<int16_t>(k - 8);
+    } else if (k == 16) {
+        f = reinterpret_cast<const void *>(&spec_last<int16_t, false>);
+    } else if (k < 24) {
+        f = narrow_kernel<int32_t>(k - 16);
+    } else if (k == 24) {
+        f = reinterpret_cast<const void *>(&spec_last<int32_t, false>);
+    } else if (k < 32) {
+        f = narrow_kernel<int32_t>(k - 24);
+    } else {
+        f = reinterpret_cast<const void *>(&spec_last<int32_t, true>);
+    }
+    int n = 0;
+    if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, 0) != hipSuccess || n <= 0) return 4;
+    return (uint32_t)n;
+}
 
 cr_status launch_spectral_batch(void *X, uint32_t layout, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc,
                                 uint64_t s0, uint32_t nchunks, unsigned long long *slots, uint32_t nslots,

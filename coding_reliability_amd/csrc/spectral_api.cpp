@@ -12,6 +12,9 @@
 namespace cr {
 
 size_t spectral_tile_count(uint32_t k);
+// Resident workgroups per CU of the last pass launch_spectral_batch runs for
+// (layout, k), from the occupancy API (4 if the query fails).
+uint32_t spectral_last_pass_per_cu(uint32_t layout, uint32_t k);
 cr_status launch_spectral_batch(void *X, uint32_t layout, uint32_t k, uint64_t A, uint32_t Ainv, uint64_t tc,
                                 uint64_t s0, uint32_t nchunks, unsigned long long *slots, uint32_t nslots,
                                 uint32_t nbins, hipStream_t stream);
@@ -108,7 +111,11 @@ bool spectral_plan(uint64_t A, uint32_t k, int cus, SpectralPlan *p) {
     p->per_batch = std::min<uint64_t>(p->chunks, std::max<uint64_t>(1, (1ull << 28) / chunk_bytes));
     p->batches = (p->chunks + p->per_batch - 1) / p->per_batch;
     p->buffer_bytes = p->per_batch * chunk_bytes;
-    p->nslots = (uint32_t)cus * 4;
+    // one resident generation of last-pass workgroups (one slot each): at
+    // k = 23 / 24 (two per CU) 4.7 % / 1.9 % faster than twice as many, at
+    // k = 32 (int32) 3 % faster than half as many; a multiple of 16, as the
+    // int16-pair pass needs
+    p->nslots = ((uint32_t)cus * spectral_last_pass_per_cu(p->layout, k) + 15) & ~15u;
     p->slot_entries = (uint64_t)p->nslots * 4 * p->nw;  // per weight: lo sum, carries, hi sum, carries
     return true;
 }
