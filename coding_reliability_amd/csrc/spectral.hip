@@ -378,8 +378,8 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
 // two values).  Since sa + 1 differs from sa in bit 0 only, parity((sa + 1) &
 // h) = parity(sa & h) ^ (h & 1).  Thread (wave w, lane l) generates e = w *
 // 4096 + l * 64 + j (j < 64): bits 0-5 in registers; bits 6-11 after a
-// transpose within the wave through LDS (rows of 65 dwords: conflict free
-// both ways); bits 12 .. B1-1 after an exchange across the waves in two
+// transpose within the wave (a lane swap, then 32 x 32 transposes through
+// LDS rows of 33 dwords); bits 12 .. B1-1 after an exchange across the waves in two
 // rounds of 32 registers.  Each chunk's result is written as int16 in the
 // layout Y[c][S_lo >> 6][x_hi][S_lo & 63] (x_hi < 1024), so that a tile of
 // the last pass -- all x_hi of 32 consecutive S_lo -- is one contiguous run of
@@ -411,7 +411,9 @@ __global__ __launch_bounds__(64 * W, 2) void spec_gen_pairs(int16_t *__restrict_
                                                             uint32_t Ainv, uint32_t tc, uint32_t s0) {
     constexpr uint32_t B1 = W == 4 ? 14 : (W == 2 ? 13 : 12);  // index bits of this pass
     constexpr int QB = 32 / W;                                  // registers per wave and exchange round
-    __shared__ uint32_t L[W * 64 * 65];  // a 64 x 64 transpose per wave (rows of 65), then the exchange
+    // per wave a 64 x 33-dword transpose buffer, then (W > 1) the exchange of
+    // W * 32 * 64 dwords: 33.8 KB at W = 4, four workgroups per CU
+    __shared__ uint32_t L[W * 64 * 33];
     const uint32_t t = threadIdx.x, l = t & 63, w = t >> 6;
     const uint32_t mask = (1u << k) - 1;
     const uint32_t G = blockIdx.x;
@@ -427,17 +429,35 @@ __global__ __launch_bounds__(64 * W, 2) void spec_gen_pairs(int16_t *__restrict_
         h = next_high(h, tc, A);
     }
     wht_pk<1, 64>(v);  // bits 0-5; |v| <= 2^6
-    // transpose within the wave through rows of 65 dwords: lane l's register
-    // j at R[65 l + j], read back by lane l' as register r from R[65 r + l'];
-    // the register index is an immediate offset and the banks, (l + j) and
-    // (r + l') mod 32, differ across the lanes of each access.  A wave's LDS
-    // accesses execute in order, so no barrier separates the two.
-    uint32_t *R = L + w * (64 * 65);
+    // transpose within the wave (afterwards lane l' holds as register r what
+    // lane r held as register l').  First v_permlane32_swap exchanges lane
+    // bit 5 with register bit 5: for j < 32 the upper 32 lanes of v[j] trade
+    // places with the lower 32 lanes of v[j + 32], so lane (a, m) (a = bit 5)
+    // then holds as register (b, r) what lane (b, m) held as register (a, r).
+    // The rest is a 32 x 32 transpose of the low five bits per register half
+    // b, through rows of 33 dwords: lane (a, m) stores register (b, r) at
+    // R[33 (32 a + r) + m] and reads it back from R[33 (32 a + m) + r]; the
+    // register index is an immediate offset and each half-wave's banks, (r +
+    // m) mod 32 up to a constant, are distinct.  A wave's LDS accesses execute
+    // in order, so no barrier separates them.  The buffer is half the size of
+    // a direct 64 x 65 transpose, which doubles the resident workgroups.
 #pragma unroll
-    for (int j = 0; j < 64; ++j) R[l * 65 + j] = v[j];
-    __builtin_amdgcn_wave_barrier();
+    for (int j = 0; j < 32; ++j) {
+        const auto sw = __builtin_amdgcn_permlane32_swap(v[j], v[j + 32], false, false);
+        v[j] = sw[0];
+        v[j + 32] = sw[1];
+    }
+    uint32_t *R = L + w * (64 * 33);
+    const uint32_t a = l >> 5, m = l & 31;
 #pragma unroll
-    for (int r = 0; r < 64; ++r) v[r] = R[(uint32_t)r * 65 + l];
+    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+        for (int r = 0; r < 32; ++r) R[(32 * a + r) * 33 + m] = v[32 * b + r];
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int r = 0; r < 32; ++r) v[32 * b + r] = R[(32 * a + m) * 33 + r];
+        __builtin_amdgcn_wave_barrier();
+    }
     wht_pk<1, 64>(v);  // bits 6-11 (v[r]: bits 6-11 = r, bits 0-5 = l); |v| <= 2^12
     if (W > 1) {
         // bits 12 .. B1-1 (the wave index): after round hr, v[32 hr + QB q +
