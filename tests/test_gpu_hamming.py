@@ -158,6 +158,45 @@ def test_batch_c3_grid_in_mc_launch(ctx):
     assert sum(rows[1]) == 1 << 30
 
 
+def test_batch_many_small_requests(ctx):
+    # 300 seeded requests of every source, k from 1 to 64, counts from 0 to
+    # 40000 (partial items, empty jobs), so both launches binary-search a few
+    # hundred jobs; each row equals the request's own call, and the lists and
+    # a sample of ranges also equal the oracle
+    rng = np.random.default_rng(20261203)
+    reqs = []
+    for _ in range(300):
+        k = int(rng.integers(1, 65))
+        src = int(rng.integers(0, 4))
+        count = int(rng.integers(0, 40001)) if rng.random() > 0.05 else 0
+        q = dict(k=k, source=src, count=count)
+        if src == cra.HAM_STRIDE:
+            q.update(first=int(rng.integers(0, 1 << 62)), stride=int(rng.integers(1, 1 << 62)))
+        elif src == cra.HAM_RANGE:
+            q.update(first=int(rng.integers(0, 1 << min(k, 62))))
+        elif src == cra.HAM_MC:
+            per = 4 if k <= 32 else 2
+            q.update(first=per * int(rng.integers(0, 1 << 30)), seed=int(rng.integers(0, 1 << 63)),
+                     stream=int(rng.integers(0, 1 << 32)))
+        else:
+            q = dict(k=k, source=src, words=rng.integers(0, 1 << 63, size=count, dtype=np.uint64))
+        reqs.append(q)
+    rows, ms = ctx.hamming_weights_batch(reqs)
+    checked = 0
+    for q, row in zip(reqs, rows):
+        q = dict(q)
+        k = q.pop("k")
+        h, _ = ctx.hamming_weights(k, **q)
+        assert row == h, (k, q.get("source"), q.get("count"))
+        if q.get("source") == cra.HAM_LIST and checked < 20:
+            assert row == O.hamming_weights_list(k, q["words"])[: len(row)]
+            checked += 1
+    assert checked == 20
+    with pytest.raises(cra.CRError):
+        ctx.hamming_weights_batch([dict(k=8)] * 4097)  # beyond 4096 requests
+    print(f"batch of 300: {ms:.3f} ms", flush=True)
+
+
 def test_batch_refusals(ctx):
     with pytest.raises(cra.CRError):
         ctx.hamming_weights_batch([])
