@@ -509,31 +509,45 @@ __global__ __launch_bounds__(64 * W, 2) void spec_gen_pairs(int16_t *__restrict_
 }
 
 // spec_last10: tile (c, sb, half) = rows x_hi < 1024 and the 32 columns S_lo =
-// sb * 64 + half * 32 + cc of chunk c (64 KB of int16).  Thread (row group g <
-// 16, column cc) loads rows g * 64 + j (buffer loads: tile base in the
-// resource, row offset a scalar), transforms x_hi bits 0-5 in registers, then
-// bits 6-9 after four exchange rounds through LDS (round p: every thread
-// stores registers 16 p .. 16 p + 15 and gathers register 16 p + g of all 16
-// groups; the two groups of a wave store to opposite 32-dword halves of a row,
-// alternating with the register index, so stores and gathers are conflict
-// free), and sums F^2 / 4 by weight |s| + |S_lo| + |S_hi| as spec_last sums F^2.
+// sb * 64 + half * 32 + cc of chunk c (64 KB of int16).  Thread (row group g
+// < 32, column pair c2 < 16) loads rows g * 32 + j (j < 32) of columns 2 c2 and
+// 2 c2 + 1 as one dword per row (buffer loads: tile base in the resource, row
+// offset a scalar), transforms x_hi bits 0-4 of both columns in registers,
+// then bits 5-9 after one exchange per column through LDS (every thread
+// stores its 32 registers, then gathers register g of the 32 row groups of its
+// column pair), and sums F^2 / 4 by weight |s| + |S_lo| + |S_hi| as spec_last
+// sums F^2.  The exchange rows are 528 dwords apart, so the two row groups of
+// each half-wave fall on opposite halves of the banks in both directions.
 // Tiles are visited so that the two column halves of a row segment go to
 // workgroups b and b + 8, which run on the same XCD (workgroups are dealt to
 // the 8 XCDs in turn) at the same time.
 constexpr uint32_t kLast10Threads = 512;
+constexpr uint32_t kLast10Pitch = 528;  // dwords between the exchange rows of two registers
+__device__ __forceinline__ void wht32(int32_t (&v)[32]) {
+#pragma unroll
+    for (int h = 1; h < 32; h <<= 1)
+#pragma unroll
+        for (int j = 0; j < 32; ++j)
+            if ((j & h) == 0) {
+                const int32_t a = v[j], b = v[j + h];
+                v[j] = a + b;
+                v[j + h] = a - b;
+            }
+}
 __global__ __launch_bounds__(kLast10Threads) void spec_last10(const int16_t *__restrict__ Y, uint32_t k, uint64_t s0,
                                                               uint32_t nchunks, unsigned long long *__restrict__ slots,
                                                               uint32_t nbins) {
-    __shared__ uint32_t E[8 * 16 * 64];  // 32 KB
-    // weight bins per column lane: bins[(w * 2 + half) * 32 + cc] (lo, hi
-    // halves of F^2): the lanes of an add never share an address
-    __shared__ unsigned long long bins[64 * 2 * 32];  // 32 KB
-    const uint32_t t = threadIdx.x, cc = t & 31, g = t >> 5;
-    for (uint32_t b = t; b < 64 * 2 * 32; b += kLast10Threads) bins[b] = 0;
+    __shared__ uint32_t E[31 * kLast10Pitch + 32 * 16];  // 67.5 KB
+    // weight bins in 8 copies: bins[(w * 2 + half) * 8 + (c2 & 7)] (lo, hi
+    // halves of F^2); the lanes of an add share an address at most 8-fold
+    __shared__ unsigned long long bins[64 * 2 * 8];  // 8 KB
+    const uint32_t t = threadIdx.x, c2 = t & 15, g = t >> 4;
+    for (uint32_t b = t; b < 64 * 2 * 8; b += kLast10Threads) bins[b] = 0;
     __syncthreads();
     const uint32_t X = 1024;                  // rows x_hi: the first pass took index bits 0 .. k-11
     const uint32_t sbits = k - 16;            // bits of sb
     const uint64_t ntiles = (uint64_t)nchunks << (k - 15);
+    const int voff = (int)((g * 32 * 64 + 2 * c2) * 2);
     for (uint64_t u = blockIdx.x; u < ntiles; u += gridDim.x) {
         const uint64_t jx = u >> 3;
         const uint32_t half = (uint32_t)(jx & 1);
@@ -542,60 +556,58 @@ __global__ __launch_bounds__(kLast10Threads) void spec_last10(const int16_t *__r
         const int16_t *tile = Y + ((uint64_t)c << k) + (uint64_t)sb * X * 64 + half * 32;
         const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<int16_t *>(tile), 0, (int)(((X - 1) * 64 + 32) * 2), 0x00020000);
-        int32_t v[64];
+        int32_t va[32], vb[32];  // columns 2 c2 and 2 c2 + 1
 #pragma unroll
-        for (int j = 0; j < 64; ++j)
-            v[j] = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(rs, (int)((g * 64 * 64 + cc) * 2), j * 128, 0);
-        wht64(v);  // x_hi bits 0-5; |v| <= 2^20
-#pragma unroll
-        for (int p = 0; p < 4; ++p) {
-            __syncthreads();
-#pragma unroll
-            for (int jj = 0; jj < 16; ++jj)
-                E[((g >> 1) * 16 + jj) * 64 + ((((g & 1) ^ (uint32_t)jj) & 1) << 5) + cc] = (uint32_t)v[16 * p + jj];
-            __syncthreads();
-#pragma unroll
-            for (int gg = 0; gg < 16; ++gg)
-                v[16 * p + gg] = (int32_t)E[(((uint32_t)gg >> 1) * 16 + g) * 64 + ((((uint32_t)gg ^ g) & 1) << 5) + cc];
+        for (int j = 0; j < 32; ++j) {
+            const uint32_t d = __builtin_amdgcn_raw_buffer_load_b32(rs, voff, j * 128, 0);
+            va[j] = (int32_t)(int16_t)(d & 0xFFFFu);
+            vb[j] = (int32_t)d >> 16;
         }
-        // v[16 p + gg]: x_hi bits 6-9 = gg, transformed bits 0-5 = 16 p + g
+        wht32(va);  // x_hi bits 0-4; |v| <= 2^19
+        wht32(vb);
+        // v[gg] afterwards: x_hi bits 5-9 = gg, transformed bits 0-4 = g
 #pragma unroll
-        for (int hb = 1; hb < 16; hb <<= 1)
+        for (int col = 0; col < 2; ++col) {
+            int32_t(&v)[32] = col ? vb : va;
+            __syncthreads();
 #pragma unroll
-            for (int q = 0; q < 64; ++q)
-                if ((q & hb) == 0) {
-                    const int32_t a = v[q], b = v[q + hb];
-                    v[q] = a + b;
-                    v[q + hb] = a - b;
-                }
-        // v = Q: F / 2 = -Q, except at S = 0 (thread 0, register 0 of the
-        // tiles with sb = 0 and half = 0), where F / 2 = 2^(k-1) - Q
-        if (sb == 0 && half == 0 && t == 0) v[0] -= 1 << (k - 1);
-        // |F / 2| <= 2^23: a thread's 64 squares stay below 2^52 in one register pair
+            for (int j = 0; j < 32; ++j) E[j * kLast10Pitch + g * 16 + c2] = (uint32_t)v[j];
+            __syncthreads();
+#pragma unroll
+            for (int gg = 0; gg < 32; ++gg) v[gg] = (int32_t)E[g * kLast10Pitch + gg * 16 + c2];
+        }
+        wht32(va);  // x_hi bits 5-9
+        wht32(vb);
+        // v = Q: F / 2 = -Q, except at S = 0 (thread 0, register 0 of column
+        // 0 of the tiles with sb = 0 and half = 0), where F / 2 = 2^(k-1) - Q
+        if (sb == 0 && half == 0 && t == 0) va[0] -= 1 << (k - 1);
+        // |F / 2| <= 2^23: a thread's 64 squares stay below 2^53 in one
+        // register pair; column 2 c2 + 1 has one more set bit than 2 c2
         unsigned long long acc[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int q = 0; q < 64; ++q) {
-            const int64_t f = v[q];
-            acc[__builtin_popcount(q >> 4) + __builtin_popcount(q & 15)] += (unsigned long long)(f * f);
+        for (int q = 0; q < 32; ++q) {
+            const int64_t fa = va[q], fb = vb[q];
+            acc[__builtin_popcount(q)] += (unsigned long long)(fa * fa);
+            acc[__builtin_popcount(q) + 1] += (unsigned long long)(fb * fb);
         }
-        const uint32_t base = (uint32_t)__popcll(s0 + c) + __popc(sb) + half + __popc(cc) + __popc(g);
+        const uint32_t base = (uint32_t)__popcll(s0 + c) + __popc(sb) + half + __popc(c2) + __popc(g);
 #pragma unroll
         for (int o_ = 0; o_ < 7; ++o_) {
-            atomicAdd(&bins[((base + o_) * 2) * 32 + cc], acc[o_] & 0xFFFFFFFFull);
-            atomicAdd(&bins[((base + o_) * 2 + 1) * 32 + cc], acc[o_] >> 32);
+            atomicAdd(&bins[((base + o_) * 2) * 8 + (c2 & 7)], acc[o_] & 0xFFFFFFFFull);
+            atomicAdd(&bins[((base + o_) * 2 + 1) * 8 + (c2 & 7)], acc[o_] >> 32);
         }
     }
     __syncthreads();
-    // the 32 column lanes' bins into the first column (one thread per (w, half))
+    // the 8 copies into the first (one thread per (w, half))
     for (uint32_t b = t; b < 2 * nbins; b += kLast10Threads) {
         unsigned long long x = 0;
-        for (uint32_t q = 0; q < 32; ++q) x += bins[b * 32 + ((q + b) & 31)];
-        bins[b * 32] = x;
+        for (uint32_t q = 0; q < 8; ++q) x += bins[b * 8 + ((q + b) & 7)];
+        bins[b * 8] = x;
     }
     __syncthreads();
     unsigned long long *slot = slots + (uint64_t)blockIdx.x * 4 * nbins;
     for (uint32_t b = t; b < 2 * nbins; b += kLast10Threads) {
-        const unsigned long long x = bins[b * 32], old = slot[2 * b];
+        const unsigned long long x = bins[b * 8], old = slot[2 * b];
         slot[2 * b] = old + x;
         if (old + x < old) slot[2 * b + 1] += 1;
     }
