@@ -99,10 +99,10 @@ bool spectral_plan(uint64_t A, uint32_t k, int cus, SpectralPlan *p) {
     // int16 (an even number per batch, starting at an even index)
     p->layout = (k >= 22 && k <= 24 && p->chunks >= 2) ? kSpectralInt16Pairs : kSpectralInt32;
     // measured seconds per transformed element: k = 22 .. 24 through int16
-    // (0.84-0.89 ps: 2^40 elements in 0.97 s at k = 24), two passes through
-    // int32 up to k = 24 (2^40 in 2.35 s), three beyond (2^38 in 1.03 s at
-    // k = 32)
-    p->sec_per_element = p->layout == kSpectralInt16Pairs ? 0.9e-12 : (k <= 24 ? 2.2e-12 : 3.8e-12);
+    // (0.81-0.84 ps: 2^40 elements in 0.89-0.92 s at k = 24), two passes
+    // through int32 up to k = 24 (2^40 in 2.35 s), three beyond (2^38 in
+    // 0.99-1.02 s at k = 32)
+    p->sec_per_element = p->layout == kSpectralInt16Pairs ? 0.84e-12 : (k <= 24 ? 2.2e-12 : 3.7e-12);
     // chunks of 2^k per launch: 256 MB of transforms in flight, about the
     // size of the last-level cache (measured at k = 24, A = 58659 through
     // int32: 2.36 s with 256 MB, 2.63 s with 128 MB, 2.67 s with 512 MB)
