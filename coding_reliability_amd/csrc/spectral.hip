@@ -24,6 +24,8 @@
 // F^2 by weight |s| + |S_lo|
 // exactly (lo/hi 32-bit halves in 64-bit per-workgroup slots).  The host
 // applies the Krawtchouk transform in 128-bit integers.
+#include <atomic>
+
 #include "cr_device.h"
 #include "cr_internal.h"
 
@@ -707,26 +709,37 @@ const void *narrow_kernel(uint32_t R) {
 size_t spectral_tile_count(uint32_t k) { return (size_t)1 << (k - 14); }
 
 uint32_t spectral_last_pass_per_cu(uint32_t layout, uint32_t k) {
+    // the kernel that launch_spectral_batch runs last, and a cache index per
+    // kernel (the query runs once per kernel and process: every gfx950 device
+    // gives the same answer)
+    if (k < 14 || k > 32) return 4;
     const void *f = nullptr;
-    int threads = (int)kThreads;
+    int threads = (int)kThreads, slot = 0;
     if (layout == kSpectralInt16Pairs) {
         f = reinterpret_cast<const void *>(&spec_last10);
         threads = (int)kLast10Threads;
     } else if (k < 16) {
         f = narrow_kernel<int16_t>(k - 8);
+        slot = 1 + (int)(k - 8);  // 7 .. 8
     } else if (k == 16) {
         f = reinterpret_cast<const void *>(&spec_last<int16_t, false>);
-    } else if (k < 24) {
-        f = narrow_kernel<int32_t>(k - 16);
+        slot = 9;
+    } else if (k < 24 || (k > 24 && k < 32)) {
+        f = narrow_kernel<int32_t>(k < 24 ? k - 16 : k - 24);
+        slot = 10 + (int)(k < 24 ? k - 16 : k - 24);  // 11 .. 17
     } else if (k == 24) {
         f = reinterpret_cast<const void *>(&spec_last<int32_t, false>);
-    } else if (k < 32) {
-        f = narrow_kernel<int32_t>(k - 24);
+        slot = 18;
     } else {
         f = reinterpret_cast<const void *>(&spec_last<int32_t, true>);
+        slot = 19;
     }
+    static std::atomic<int> cache[20];
+    const int known = cache[slot].load(std::memory_order_relaxed);
+    if (known > 0) return (uint32_t)known;
     int n = 0;
     if (!f || hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, threads, 0) != hipSuccess || n <= 0) return 4;
+    cache[slot].store(n, std::memory_order_relaxed);
     return (uint32_t)n;
 }
 
