@@ -181,7 +181,7 @@ struct SpectralPlan {
     double sec_per_element = 0;   // measured device seconds per transformed element
 };
 // Pass structures of the spectral path: index bits 0-15 (or 0-7 for k <= 16)
-// then the rest through an int32 (int16) buffer; or, for k = 22 .. 24 and at
+// then the rest through an int32 (int16) buffer; or, for k = 17 .. 24 and at
 // least two transforms, bits 0 .. k-11 of two transforms at once then the
 // last 10 bits, through int16.
 constexpr uint32_t kSpectralInt32 = 0, kSpectralInt16Pairs = 1;

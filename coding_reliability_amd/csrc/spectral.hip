@@ -362,7 +362,7 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
 }
 
 // ---------------------------------------------------------------------------
-// k = 22 .. 24 through int16 (2 bytes of traffic per element each way): the
+// k = 17 .. 24 through int16 (2 bytes of traffic per element each way): the
 // first pass transforms index bits 0 .. B1-1, B1 = k - 10 (values within
 // +-2^B1 fit int16), of two chunks at once, the last pass the remaining 10
 // bits.
@@ -373,16 +373,18 @@ __global__ __launch_bounds__(kThreads) void spec_last(const In *__restrict__ X, 
 // (Q(0) - 2^(k-1))^2.  The last pass sums F^2 / 4 and the host multiplies the
 // weight sums by 4.  After B1 <= 14 stages |Q| <= 2^14, within int16.
 //
-// spec_gen_pairs<W>: workgroup (G, y) of W waves (W = 2^(B1-12)) takes the
-// 2^B1 elements x_lo = G * 2^B1 + e of chunks sa = s0 + 2y (even) and sa + 1,
+// spec_gen_pairs<W, B1>: workgroup (G, y) of W waves (W = 2^(B1-12) for k =
+// 22 .. 24, W = 1 below) takes the 2^SPAN elements G * 2^SPAN + e (SPAN =
+// max(B1, 12)) of chunks sa = s0 + 2y (even) and sa + 1,
 // each dword of its working set holding the two chunks' values as int16
 // halves (packed v_pk_add_u16 / v_pk_sub_i16 butterflies: one instruction per
 // two values).  Since sa + 1 differs from sa in bit 0 only, parity((sa + 1) &
 // h) = parity(sa & h) ^ (h & 1).  Thread (wave w, lane l) generates e = w *
-// 4096 + l * 64 + j (j < 64): bits 0-5 in registers; bits 6-11 after a
-// transpose within the wave (a lane swap, then 32 x 32 transposes through
-// LDS rows of 33 dwords); bits 12 .. B1-1 after an exchange across the waves in two
-// rounds of 32 registers.  Each chunk's result is written as int16 in the
+// 4096 + l * 64 + j (j < 64): bits 0-5 in registers; bits 6 .. min(B1, 12)-1
+// after a transpose within the wave (a lane swap, then 32 x 32 transposes
+// through LDS rows of 33 dwords); bits 12 .. B1-1 after an exchange across the
+// waves in two rounds of 32 registers.  For B1 < 12 the untransformed bits B1
+// .. 11 are the low bits of the row x_hi.  Each chunk's result is written as int16 in the
 // layout Y[c][S_lo >> 6][x_hi][S_lo & 63] (x_hi < 1024), so that a tile of
 // the last pass -- all x_hi of 32 consecutive S_lo -- is one contiguous run of
 // 64-byte rows.
@@ -408,11 +410,14 @@ __device__ __forceinline__ void wht_pk(uint32_t (&v)[64]) {
             }
 }
 
-template <int W>
+template <int W, int B1>
 __global__ __launch_bounds__(64 * W, 2) void spec_gen_pairs(int16_t *__restrict__ Y, uint32_t k, uint32_t A,
                                                             uint32_t Ainv, uint32_t tc, uint32_t s0) {
-    constexpr uint32_t B1 = W == 4 ? 14 : (W == 2 ? 13 : 12);  // index bits of this pass
-    constexpr int QB = 32 / W;                                  // registers per wave and exchange round
+    // B1 index bits transformed here; a workgroup covers 2^SPAN elements, so
+    // for W = 1 and B1 < 12 its top 12 - B1 bits are left to the last pass
+    constexpr uint32_t SPAN = W == 4 ? 14 : (W == 2 ? 13 : 12);
+    static_assert(W == 1 ? (B1 >= 7 && B1 <= 12) : B1 == (int)SPAN, "spec_gen_pairs shape");
+    constexpr int QB = 32 / W;  // registers per wave and exchange round
     // per wave a 64 x 33-dword transpose buffer, then (W > 1) the exchange of
     // W * 32 * 64 dwords: 33.8 KB at W = 4, four workgroups per CU
     __shared__ uint32_t L[W * 64 * 33];
@@ -420,7 +425,7 @@ __global__ __launch_bounds__(64 * W, 2) void spec_gen_pairs(int16_t *__restrict_
     const uint32_t mask = (1u << k) - 1;
     const uint32_t G = blockIdx.x;
     const uint32_t sa = s0 + 2 * blockIdx.y;
-    const uint32_t x0 = (G << B1) | (w << 12) | (l << 6);
+    const uint32_t x0 = (G << SPAN) | (w << 12) | (l << 6);
     const uint32_t i = (uint32_t)((uint64_t)x0 * Ainv) & mask;
     uint32_t h = (uint32_t)(((uint64_t)A * i) >> k);
     uint32_t v[64];
@@ -460,7 +465,8 @@ __global__ __launch_bounds__(64 * W, 2) void spec_gen_pairs(int16_t *__restrict_
         for (int r = 0; r < 32; ++r) v[32 * b + r] = R[(32 * a + m) * 33 + r];
         __builtin_amdgcn_wave_barrier();
     }
-    wht_pk<1, 64>(v);  // bits 6-11 (v[r]: bits 6-11 = r, bits 0-5 = l); |v| <= 2^12
+    // bits 6 .. min(B1, 12) - 1 (v[r]: bits 6-11 = r, bits 0-5 = l); |v| <= 2^min(B1, 12)
+    wht_pk<1, (B1 >= 12 ? 64 : (1 << (B1 - 6)))>(v);
     if (W > 1) {
         // bits 12 .. B1-1 (the wave index): after round hr, v[32 hr + QB q +
         // rr] holds register 32 hr + QB w + rr of wave q
@@ -492,15 +498,19 @@ __global__ __launch_bounds__(64 * W, 2) void spec_gen_pairs(int16_t *__restrict_
     rb.z = (int)(2u << k);
     rb.w = 0x00020000;
     const uint32_t wu = __builtin_amdgcn_readfirstlane(w);
-    const int voff = (int)((G * 64 + l) * 2);
+    // the workgroup's first row x_hi; for W = 1 and B1 < 12 register r holds
+    // S_lo >> 6 = r mod 2^(B1-6) of row x_hi + (r >> (B1-6))
+    const int voff = (int)(((G << (SPAN - B1)) * 64 + l) * 2);
 #pragma unroll
     for (int hr = 0; hr < 2; ++hr)
 #pragma unroll
         for (int q = 0; q < W; ++q)
 #pragma unroll
             for (int rr = 0; rr < QB; ++rr) {
-                const uint32_t sb = ((uint32_t)q << 6) | (32u * hr + QB * wu + rr);  // S_lo >> 6
-                const int soff = (int)(sb * X * 128u);                             // bytes of sb * X rows of 64
+                const uint32_t r = 32u * hr + QB * wu + rr;
+                const uint32_t sb = W == 1 ? (r & ((1u << (B1 - 6 < 6 ? B1 - 6 : 6)) - 1)) : (((uint32_t)q << 6) | r);
+                const uint32_t xr = W == 1 && B1 < 12 ? r >> (B1 - 6) : 0;
+                const int soff = (int)((sb * X + xr) * 128u);  // bytes of sb * X + xr rows of 64
                 const uint32_t x = v[32 * hr + QB * q + rr];
                 __builtin_amdgcn_raw_buffer_store_b16((uint16_t)(x & 0xFFFFu), ra, voff, soff, 0);
                 asm volatile("buffer_store_short_d16_hi %0, %1, %2, %3 offen"
@@ -752,20 +762,31 @@ cr_status launch_spectral_batch(void *X, uint32_t layout, uint32_t k, uint64_t A
         return fail(CR_ERR_UNSUPPORTED, "spectral path: multiplier or transform count beyond 32 bits");
     if (layout == kSpectralInt16Pairs) {
         // X: nchunks * 2^k int16 after index bits 0 .. k-11, in pairs of transforms
-        if (k < 22 || k > 24 || (nchunks & 1) || (s0 & 1) || (nslots & 15))
+        // spec_last10 pairs the tiles in groups of 16 (8 row blocks x 2
+        // column halves): the tile count must be a multiple of 16
+        if (k < 17 || k > 24 || (nchunks & 1) || (s0 & 1) || (nslots & 15) ||
+            (((uint64_t)nchunks << (k - 15)) & 15))
             return fail(CR_ERR_INVALID_ARGUMENT, "spectral int16 pairs: k=%u, %u transforms from %llu, %u slots", k,
                         nchunks, (unsigned long long)s0, nslots);
         int16_t *y = static_cast<int16_t *>(X);
-        const dim3 grid(1024, nchunks / 2);
-        if (k == 24)
-            hipLaunchKernelGGL(spec_gen_pairs<4>, grid, dim3(256), 0, stream, y, k, (uint32_t)A, Ainv, (uint32_t)tc,
-                               (uint32_t)s0);
-        else if (k == 23)
-            hipLaunchKernelGGL(spec_gen_pairs<2>, grid, dim3(128), 0, stream, y, k, (uint32_t)A, Ainv, (uint32_t)tc,
-                               (uint32_t)s0);
-        else
-            hipLaunchKernelGGL(spec_gen_pairs<1>, grid, dim3(64), 0, stream, y, k, (uint32_t)A, Ainv, (uint32_t)tc,
-                               (uint32_t)s0);
+        // k = 24 / 23: four / two waves over index bits 0-13 / 0-12; k <= 22:
+        // one wave over bits 0 .. k-11 of 2^12 elements (1024 rows each)
+        const dim3 grid(k >= 23 ? 1024 : 1u << (k - 12), nchunks / 2);
+        const uint32_t a32 = (uint32_t)A, tc32 = (uint32_t)tc, s32 = (uint32_t)s0;
+#define CR_GEN(WW, BB)                                                                                  \
+    hipLaunchKernelGGL((spec_gen_pairs<WW, BB>), grid, dim3(64 * WW), 0, stream, y, k, a32, Ainv, tc32, s32); \
+    break;
+        switch (k) {
+        case 24: CR_GEN(4, 14)
+        case 23: CR_GEN(2, 13)
+        case 22: CR_GEN(1, 12)
+        case 21: CR_GEN(1, 11)
+        case 20: CR_GEN(1, 10)
+        case 19: CR_GEN(1, 9)
+        case 18: CR_GEN(1, 8)
+        default: CR_GEN(1, 7)
+        }
+#undef CR_GEN
         CR_HIP(hipGetLastError());
         hipLaunchKernelGGL(spec_last10, dim3(nslots), dim3(kLast10Threads), 0, stream, (const int16_t *)y, k, s0,
                            nchunks, slots, nbins);

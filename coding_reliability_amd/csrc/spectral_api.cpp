@@ -95,11 +95,13 @@ bool spectral_plan(uint64_t A, uint32_t k, int cus, SpectralPlan *p) {
     p->Ainv = inverse_mod_2_32((uint32_t)Ao) & (uint32_t)(N - 1);
     p->tc = (Ao * p->Ainv - 1) >> k;  // Ao * Ainv = 1 + tc * 2^k (below 2^56)
     p->chunks = 1ull << (n - k);
-    // k = 22 .. 24 with at least two transforms: pairs of transforms through
-    // int16 (an even number per batch, starting at an even index)
-    p->layout = (k >= 22 && k <= 24 && p->chunks >= 2) ? kSpectralInt16Pairs : kSpectralInt32;
-    // measured seconds per transformed element: k = 22 .. 24 through int16
-    // (0.81-0.84 ps: 2^40 elements in 0.89-0.92 s at k = 24), two passes
+    // k = 17 .. 24 with at least two transforms (four at k = 17, whose last
+    // pass needs 16 tiles of 2^15 elements per batch): pairs of transforms
+    // through int16 (an even number per batch, starting at an even index)
+    p->layout = (k >= 17 && k <= 24 && p->chunks >= (k == 17 ? 4u : 2u)) ? kSpectralInt16Pairs : kSpectralInt32;
+    // measured seconds per transformed element: k = 17 .. 24 through int16
+    // (0.80-0.84 ps: 2^40 elements in 0.89-0.92 s at k = 24, 2^37 in 109 ms
+    // at k = 21, 2^34 in 13.8 ms at k = 18), two passes
     // through int32 up to k = 24 (2^40 in 2.35 s), three beyond (2^38 in
     // 0.99-1.02 s at k = 32)
     p->sec_per_element = p->layout == kSpectralInt16Pairs ? 0.84e-12 : (k <= 24 ? 2.2e-12 : 3.7e-12);

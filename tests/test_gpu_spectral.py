@@ -42,8 +42,12 @@ def test_recorded_runs_other_k(ctx, k, nruns):
         assert report(k, done[r["A"]], len(r["counts"])) == r["counts"], r["src"]
 
 
-@pytest.mark.parametrize("A,k", [(61, 17), (641, 19), (61, 21), (3, 23), (61, 23)])
-def test_narrow_last_pass_matches_pair_kernel(ctx, A, k):
+@pytest.mark.parametrize("A,k", [(61, 17), (641, 19), (61, 21), (3, 23), (61, 23), (3, 17), (58659, 18),
+                                 (1021, 20), (3, 21)])
+def test_int16_pairs_k17_to_23_match_pair_kernel(ctx, A, k):
+    # k = 17 .. 21: the one-wave first pass over bits 0 .. k-11 (its top 22 - k
+    # index bits of each 4096-element block left as rows), the 10-bit last
+    # pass; A = 3 at k = 17 has the fewest transforms that path takes (four)
     raw_s, _ = ctx.an_pairs_spectral(A, k)
     raw_p, _ = ctx.an_pairs(A, 1 << k)
     assert raw_s == raw_p
@@ -69,9 +73,11 @@ def test_k25_multi_transform_matches_pair_kernel(ctx):
     assert raw_s == raw_p
 
 
-@pytest.mark.parametrize("k", [25, 26, 27, 28, 29, 30, 31])
-def test_closed_form_k25_to_31(ctx, k):
-    # A = 1: D(d) = 2^k C(k, d) (the middle pass and the narrow last pass)
+@pytest.mark.parametrize("k", [17, 18, 19, 20, 21, 25, 26, 27, 28, 29, 30, 31])
+def test_closed_form_k17_to_31(ctx, k):
+    # A = 1: D(d) = 2^k C(k, d); one transform, so k = 17 .. 21 take the
+    # int32 passes (the 16-bit first pass and the narrow last pass), k >= 25
+    # the middle pass too
     import math
     raw, _ = ctx.an_pairs_spectral(1, k)
     assert raw[1:k + 1] == [(1 << (k - 1)) * math.comb(k, d) for d in range(1, k + 1)]
