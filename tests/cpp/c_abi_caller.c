@@ -38,6 +38,18 @@ int main(void) {
         return 4;
     }
     print_row("hamming16", w, 23);
+
+    /* the same request and a (13,8) window in one batched call */
+    cr_ham_request batch[2] = {{16, CR_HAM_RANGE, 0, 65536, 0, NULL, 0, 0, 0},
+                               {8, CR_HAM_RANGE, 100, 100, 0, NULL, 0, 0, 0}};
+    uint64_t bw[2 * 23];
+    s = cr_hamming_weights_batch(ctx, batch, 2, bw, 23, &ms);
+    if (s != CR_OK) {
+        fprintf(stderr, "cr_hamming_weights_batch: %s\n", cr_last_error());
+        return 6;
+    }
+    print_row("batch_hamming16", bw, 23);
+    print_row("batch_hamming8_100_200", bw + 23, 23);
     /* AN Monte Carlo, A = 61, n = 10, seed 7: sample ranges [0, 100) and [100, 300) */
     const uint64_t bounds[3] = {0, 100, 300};
     uint64_t hists[2 * 65];

@@ -28,6 +28,8 @@ def test_c_caller_matches_oracle(solutions):
     # reported form (counts[0] = 2^k, counts[i] = 2 raw[i]) = solutions.h:26
     assert [256] + [2 * x for x in raw[1:]] == solutions["solution_an8_A61"][:15]
     assert [int(x) for x in rows["hamming16"].split()] == O.hamming_weights(16)[:23]
+    assert rows["batch_hamming16"] == rows["hamming16"]
+    assert [int(x) for x in rows["batch_hamming8_100_200"].split()] == O.hamming_weights(8, 100, 200)[:23]
     m100, m300 = O.an_mc(61, 10, 100, 7), O.an_mc(61, 10, 300, 7)
     assert [int(x) for x in rows["an_mc_0_100"].split()] == m100[:65]
     assert [int(x) for x in rows["an_mc_100_300"].split()] == [b - a for a, b in zip(m100, m300)][:65]
